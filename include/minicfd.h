@@ -1,0 +1,175 @@
+/*
+ * minicfd.h -- C ABI of the MI355X-native high-order residual and
+ * time-stepping library (libminicfd.so).
+ *
+ * Every entry point replaces one piece of the reference's hot path behind
+ * the mini::temporal::System contract (include/mini/temporal/ode.hpp:17-48)
+ * and the mini::spatial::FiniteElement family (include/mini/spatial/fem.hpp).
+ * The reference is a header-only C++ library with no foreign-function layer
+ * of its own; these functions are what a C/ctypes/cgo binding of that
+ * surface binds.  Each declaration names the reference interface it replaces.
+ *
+ * Conventions
+ *   - every function returns 0 on success and a nonzero MCFD_E* code on
+ *     failure; mcfd_last_error() describes the last failure of the calling
+ *     thread (no exception crosses the ABI; the reference only asserts,
+ *     include/mini/algebra/root.hpp:67 throws);
+ *   - a solution/residual "column" has the reference's layout
+ *     column[(cell * N + node) * 5 + k] with k = (rho, rho u, rho v, rho w, E)
+ *     (include/mini/mesh/part.hpp:754-759, polynomial/expansion.hpp:83-104);
+ *   - host pointers are caller-owned; the handle owns all device memory;
+ *   - one handle per host thread; calls on a handle are stream-ordered on the
+ *     handle's own HIP stream (the reference is single-threaded per rank).
+ */
+#ifndef MINICFD_H_
+#define MINICFD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes */
+#define MCFD_OK 0
+#define MCFD_EINVAL 1   /* invalid argument or mesh */
+#define MCFD_EHIP 2     /* HIP runtime error */
+#define MCFD_ESTATE 3   /* call not valid in the handle's current state */
+#define MCFD_ENOTSUP 4  /* configuration outside what the library implements */
+
+/* ---- cell types (CGNS element types used by the reference demos) */
+#define MCFD_HEXA_8 0
+#define MCFD_TETRA_4 1
+
+/* ---- spatial schemes
+ * FR_GENERAL: nodal flux reconstruction, spatial/fr/general.hpp (solution
+ *   points = tensor-product quadrature nodes; Gauss-Legendre or Lobatto).
+ * DG_GENERAL: modal DG on the per-cell orthonormal basis,
+ *   spatial/dg/general.hpp with polynomial/projection.hpp. */
+#define MCFD_SCHEME_FR_GENERAL 0
+#define MCFD_SCHEME_DG_GENERAL 1
+
+#define MCFD_NODES_LEGENDRE 0
+#define MCFD_NODES_LOBATTO 1
+
+/* ---- Riemann solvers, riemann/euler/{exact,hllc}.hpp; Roe follows
+ * python/riemann.py:405-451 extended to 3-D in the rotated frame. */
+#define MCFD_RIEMANN_EXACT 0
+#define MCFD_RIEMANN_HLLC 1
+#define MCFD_RIEMANN_ROE 2
+
+/* ---- precision forms */
+#define MCFD_PRECISION_FP64 0
+#define MCFD_PRECISION_MIXED 1
+
+/* ---- creation flags
+ * HOST_ONLY: build connectivity and tables on the host only (no device
+ * memory, no HIP calls); compute entry points then return MCFD_ESTATE.
+ * Used to inspect the face construction on machines without a GPU. */
+#define MCFD_FLAG_HOST_ONLY 1
+
+/* ---- boundary conditions, spatial/fem.hpp:196-222 (constant states) */
+#define MCFD_BC_SMART 0               /* SetSmartBoundary(name, f) */
+#define MCFD_BC_INVISCID_WALL 1       /* SetInviscidWall(name) */
+#define MCFD_BC_SUPERSONIC_OUTLET 2   /* SetSupersonicOutlet(name) */
+#define MCFD_BC_SUPERSONIC_INLET 3    /* SetSupersonicInlet(name, f) */
+
+typedef struct mcfd_solver mcfd_solver;
+
+/* Mesh in the reference's input form (CGNS connectivity, one rank).
+ * Replaces mesh::part::Part construction from CGNS + METIS partition files
+ * (mesh/part.hpp:453-476).  Negative boundary tags mark periodic faces,
+ * which are paired across the box by `period` (no reference counterpart). */
+typedef struct {
+  int32_t cell_type;            /* MCFD_HEXA_8 or MCFD_TETRA_4 */
+  int64_t n_nodes;
+  const double *coords;         /* [n_nodes][3] */
+  int64_t n_cells;
+  const int64_t *cell_nodes;    /* [n_cells][npe], CGNS order, 0-based */
+  int64_t n_bfaces;
+  const int64_t *bface_nodes;   /* [n_bfaces][face npe] */
+  const int32_t *bface_tags;    /* [n_bfaces]: BC table index, < 0 periodic */
+  double period[3];             /* box length of periodic directions, else 0 */
+} mcfd_mesh_desc;
+
+typedef struct {
+  int32_t scheme;       /* MCFD_SCHEME_* */
+  int32_t degree;       /* polynomial degree p */
+  int32_t nodes;        /* MCFD_NODES_* (FR_GENERAL) */
+  int32_t riemann;      /* MCFD_RIEMANN_* */
+  int32_t precision;    /* MCFD_PRECISION_* */
+  int32_t flags;        /* MCFD_FLAG_* */
+  double c_next;        /* Vincent c_next: 1 = DG, p/(2p+1) = Huynh g2 */
+} mcfd_scheme_desc;
+
+typedef struct {
+  int32_t kind;         /* MCFD_BC_* */
+  int32_t reserved;
+  double state[5];      /* conservative state for SMART / SUPERSONIC_INLET */
+} mcfd_bc_desc;
+
+/* Build a solver.  Replaces the constructors FiniteElement(Part*)
+ * (spatial/fem.hpp:95-115), fr::General(Part*, c_next)
+ * (spatial/fr/general.hpp:218-231) and dg::General(Part*)
+ * (spatial/dg/general.hpp:38-40) plus the BC registration
+ * (spatial/fem.hpp:196-222). */
+int mcfd_create(const mcfd_mesh_desc *mesh, const mcfd_scheme_desc *scheme,
+                const mcfd_bc_desc *bcs, int32_t n_bcs, mcfd_solver **out);
+int mcfd_destroy(mcfd_solver *h);
+const char *mcfd_last_error(void);
+
+/* Sizes: cells, unknowns per cell per component (N), column length. */
+int mcfd_sizes(const mcfd_solver *h, int64_t *n_cells, int32_t *n_per_cell,
+               int64_t *column_size);
+
+/* System::SetTime (temporal/ode.hpp:27, spatial/fem.hpp:225-227). */
+int mcfd_set_time(mcfd_solver *h, double t);
+/* System::SetSolutionColumn (ode.hpp:33, fem.hpp:228-236); host column. */
+int mcfd_set_solution_column(mcfd_solver *h, const double *column);
+/* System::GetSolutionColumn (ode.hpp:40, fem.hpp:237-247); host column. */
+int mcfd_get_solution_column(mcfd_solver *h, double *column);
+/* System::GetResidualColumn (ode.hpp:47, fem.hpp:248-258 with the FR
+ * det(J) scaling of fr/general.hpp:249-268); host column. */
+int mcfd_get_residual_column(mcfd_solver *h, double *column);
+
+/* RungeKutta<3>::Update (temporal/rk.hpp:31-49) fused on the device: three
+ * residual evaluations with the Shu-Osher combinations in the reference's
+ * operation order; the state stays resident in HBM. */
+int mcfd_rk3_update(mcfd_solver *h, double t, double dt);
+/* One stage of the above: stage 1 U1 = U0 + dt R(U0); stage 2
+ * U2 = (3 U0 + (U1 + dt R(U1))) / 4; stage 3 U0 <- (2 (U2 + dt R(U2)) + U0) / 3. */
+int mcfd_rk3_stage(mcfd_solver *h, int32_t stage, double t, double dt);
+
+/* FiniteElement::GetTimeStep (spatial/fem.hpp:417-435) with an explicit
+ * CFL number instead of the cfl_ table (fem.hpp:26-37). */
+int mcfd_get_time_step(mcfd_solver *h, double cfl, double dt_guess, double *dt);
+
+/* Coordinates of the points where an initial condition is sampled:
+ * FR_GENERAL: solution points ([n_cells][N][3], polynomial/hexahedron.hpp:646-652).
+ * DG_GENERAL: cell quadrature points ([n_cells][Nq][3], projection.hpp:28-43). */
+int mcfd_sample_points(const mcfd_solver *h, int32_t *n_per_cell, double *xyz);
+/* Polynomial::Approximate (fem.hpp:147-154): build the solution from values
+ * at the sample points ([n_cells][n_per_cell][5]) and make it current. */
+int mcfd_approximate(mcfd_solver *h, const double *values);
+
+/* Connectivity export for bit-exact comparison with the reference's face
+ * construction (mesh/part.hpp:932-975,1456-1549) and FR flux-point caches
+ * (spatial/fr/general.hpp:85-211): per face holder, sharer (-1 on the
+ * boundary), local face ids, and per face point the FluxPointCache::ijk of
+ * the holder and of the sharer. */
+int mcfd_face_count(const mcfd_solver *h, int64_t *n_faces, int64_t *n_interior);
+int mcfd_faces(const mcfd_solver *h, int64_t *holder, int64_t *sharer,
+               int32_t *holder_face, int32_t *sharer_face, int32_t *fp_ijk);
+
+/* Stream synchronisation and per-kernel timing of n RK3 steps with HIP
+ * events on the handle's stream: total ms, and ms of the stage kernels. */
+int mcfd_synchronize(mcfd_solver *h);
+int mcfd_time_rk3(mcfd_solver *h, double dt, int32_t n_steps,
+                  double *total_ms, double *stage_kernel_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MINICFD_H_ */

@@ -1,0 +1,225 @@
+// Oracle (test infrastructure): C entry points used by tests/ through ctypes,
+// by the smoke check and by the CPU-baseline leg of bench.py.  Nothing in the
+// product links against this library.
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <vector>
+
+#include "core.hpp"
+#include "fr.hpp"
+#include "mesh.hpp"
+
+using namespace oracle;
+
+namespace {
+
+struct FrHandle {
+  Mesh mesh;
+  std::unique_ptr<FrScheme> scheme;
+};
+
+thread_local std::string g_error;
+
+template <class F>
+int Guard(F &&f) {
+  try {
+    f();
+    return 0;
+  } catch (std::exception const &e) {
+    g_error = e.what();
+    return 1;
+  }
+}
+
+MeshInput MakeInput(int cell_type, int64_t n_nodes, double const *coords, int64_t n_cells,
+    int64_t const *cells, int64_t n_bfaces, int64_t const *bface_nodes, int32_t const *bface_tags,
+    double const *period) {
+  return MeshInput{ cell_type, n_nodes, coords, n_cells, cells, n_bfaces, bface_nodes, bface_tags, period };
+}
+
+std::vector<BcEntry> MakeBcs(int n_bc, int32_t const *kinds, double const *states) {
+  std::vector<BcEntry> bcs(n_bc);
+  for (int b = 0; b < n_bc; ++b) {
+    bcs[b].kind = kinds[b];
+    for (int k = 0; k < 5; ++k) bcs[b].state[k] = states[5 * b + k];
+  }
+  return bcs;
+}
+
+// temporal/rk.hpp:31-49 with temporal/ode.hpp:121-128, same operation order.
+template <class Residual>
+void Rk3Step(size_t size, double *u, double dt, Residual &&residual) {
+  std::vector<double> u_curr(u, u + size), r(size), u_next(size);
+  auto next_solution = [&](double const *state, double *out) {  // Euler::NextSolution
+    residual(state, r.data());
+    for (size_t i = 0; i < size; ++i) { r[i] *= dt; out[i] = state[i] + r[i]; }
+  };
+  std::vector<double> u1(size), t(size);
+  next_solution(u, u1.data());
+  for (size_t i = 0; i < size; ++i) { u_next[i] = u_curr[i]; u_next[i] *= 3; }
+  next_solution(u1.data(), t.data());
+  for (size_t i = 0; i < size; ++i) { u_next[i] += t[i]; u_next[i] /= 4; }
+  next_solution(u_next.data(), t.data());
+  for (size_t i = 0; i < size; ++i) { t[i] *= 2; t[i] += u_curr[i]; t[i] /= 3; u[i] = t[i]; }
+}
+
+}  // namespace
+
+extern "C" {
+
+char const *oracle_last_error() { return g_error.c_str(); }
+
+// ---------------------------------------------------------------- building blocks
+int oracle_quadrature(int lobatto, int q, double *x, double *w) {
+  return Guard([&] { if (lobatto) LobattoRule(q, x, w); else LegendreRule(q, x, w); });
+}
+// out = { g_R(s), g_L(s), g_R'(s), g_L'(s) }
+void oracle_vincent(int degree, double c_next, double s, double *out) {
+  Vincent v(degree, c_next);
+  out[0] = v.RightValue(s); out[1] = v.LeftValue(s);
+  out[2] = v.RightDerivative(s); out[3] = v.LeftDerivative(s);
+}
+// Lagrange basis on `q` nodes x: values and derivatives of all basis functions at s.
+void oracle_lagrange(int q, double const *x, double s, double *values, double *derivs) {
+  Lagrange1D l;
+  l.q = q;
+  for (int i = 0; i < q; ++i) l.x[i] = x[i];
+  for (int i = 0; i < q; ++i) { values[i] = l.Value(i, s); derivs[i] = l.Derivative(i, s); }
+}
+void oracle_conservative_to_primitive(double const *cv, double *pv) {
+  Val5 c, p;
+  std::memcpy(c.data(), cv, 40);
+  p = ConservativeToPrimitive(c);
+  std::memcpy(pv, p.data(), 40);
+}
+void oracle_primitive_to_conservative(double const *pv, double *cv) {
+  Val5 c, p;
+  std::memcpy(p.data(), pv, 40);
+  c = PrimitiveToConservative(p);
+  std::memcpy(cv, c.data(), 40);
+}
+void oracle_flux_matrix(double const *cv, double *f15) {
+  Val5 c;
+  std::memcpy(c.data(), cv, 40);
+  Flux53 f = GetFluxMatrix(c);
+  for (int k = 0; k < 5; ++k) for (int d = 0; d < 3; ++d) f15[k * 3 + d] = f[k][d];
+}
+// 1-D exact solver on normal-frame primitives: the t-axis primitive state.
+void oracle_exact_primitive(double const *pl, double const *pr, double *out) {
+  Val5 l, r;
+  std::memcpy(l.data(), pl, 40);
+  std::memcpy(r.data(), pr, 40);
+  ExactSolver s;
+  Val5 o = s.PrimitiveUpwind(l, r);
+  std::memcpy(out, o.data(), 40);
+}
+// Rotated upwind flux of conservative states, frame = {a, b, c} row-wise.
+void oracle_riemann_flux(int kind, double const *frame9, double const *cl, double const *cr, double *flux) {
+  Frame f;
+  for (int d = 0; d < 3; ++d) { f.a[d] = frame9[d]; f.b[d] = frame9[3 + d]; f.c[d] = frame9[6 + d]; }
+  Val5 l, r;
+  std::memcpy(l.data(), cl, 40);
+  std::memcpy(r.data(), cr, 40);
+  RotatedEuler re;
+  re.kind = RiemannKind(kind);
+  Val5 o = re.GetFluxUpwind(f, l, r);
+  std::memcpy(flux, o.data(), 40);
+}
+void oracle_wall_flux(double const *frame9, double const *c, double *flux) {
+  Frame f;
+  for (int d = 0; d < 3; ++d) { f.a[d] = frame9[d]; f.b[d] = frame9[3 + d]; f.c[d] = frame9[6 + d]; }
+  Val5 u;
+  std::memcpy(u.data(), c, 40);
+  RotatedEuler re;
+  re.kind = kHllc;
+  Val5 o = re.GetFluxOnInviscidWall(f, u);
+  std::memcpy(flux, o.data(), 40);
+}
+
+// ---------------------------------------------------------------- nodal FR on hexahedra
+int oracle_fr_create(int64_t n_nodes, double const *coords, int64_t n_cells, int64_t const *cells,
+    int64_t n_bfaces, int64_t const *bface_nodes, int32_t const *bface_tags, double const *period,
+    int q, int lobatto, double c_next, int riemann, int n_bc, int32_t const *bc_kinds,
+    double const *bc_states, void **out) {
+  return Guard([&] {
+    auto h = std::make_unique<FrHandle>();
+    h->mesh = BuildHexMesh(MakeInput(kHexa8, n_nodes, coords, n_cells, cells, n_bfaces, bface_nodes,
+        bface_tags, period), q, lobatto);
+    h->scheme = std::make_unique<FrScheme>(&h->mesh, c_next, riemann, MakeBcs(n_bc, bc_kinds, bc_states));
+    *out = h.release();
+  });
+}
+void oracle_fr_destroy(void *h) { delete static_cast<FrHandle *>(h); }
+int64_t oracle_fr_n_faces(void *h, int64_t *n_interior) {
+  auto *p = static_cast<FrHandle *>(h);
+  *n_interior = p->mesh.n_interior;
+  return p->mesh.faces.size();
+}
+// Per face: holder, sharer, local face ids, and the FluxPointCache::ijk of
+// every face point on both sides (-1 on the sharer side of boundary faces).
+void oracle_fr_faces(void *h, int64_t *holder, int64_t *sharer, int32_t *holder_face,
+    int32_t *sharer_face, int32_t *fp_ijk, double *frames) {
+  auto *p = static_cast<FrHandle *>(h);
+  auto &s = *p->scheme;
+  int nfp = s.q * s.q;
+  for (size_t f = 0; f < p->mesh.faces.size(); ++f) {
+    auto const &face = p->mesh.faces[f];
+    holder[f] = face.holder; sharer[f] = face.sharer;
+    holder_face[f] = face.holder_face; sharer_face[f] = face.sharer_face;
+    for (int i = 0; i < nfp; ++i) {
+      auto const &ix = s.fp_index[f * nfp + i];
+      fp_ijk[(f * nfp + i) * 2] = s.FluxPointNode(face.holder_face, ix[0], ix[1]);
+      fp_ijk[(f * nfp + i) * 2 + 1] = face.sharer >= 0 ? s.FluxPointNode(face.sharer_face, ix[2], ix[3]) : -1;
+      auto const &fr = face.frames[i];
+      for (int d = 0; d < 3; ++d) {
+        frames[(f * nfp + i) * 9 + d] = fr.a[d];
+        frames[(f * nfp + i) * 9 + 3 + d] = fr.b[d];
+        frames[(f * nfp + i) * 9 + 6 + d] = fr.c[d];
+      }
+    }
+  }
+}
+// Global coordinates of every solution point, [cell][ijk][xyz].
+void oracle_fr_node_coords(void *h, double *xyz) {
+  auto *p = static_cast<FrHandle *>(h);
+  size_t n = p->scheme->n, i = 0;
+  for (auto const &cell : p->mesh.cells)
+    for (size_t q = 0; q < n; ++q, ++i)
+      for (int d = 0; d < 3; ++d) xyz[3 * i + d] = cell.qp[q][d];
+}
+// Quadrature weights (global) of every solution point, [cell][ijk].
+void oracle_fr_node_weights(void *h, double *w) {
+  auto *p = static_cast<FrHandle *>(h);
+  size_t n = p->scheme->n, i = 0;
+  for (auto const &cell : p->mesh.cells)
+    for (size_t q = 0; q < n; ++q) w[i++] = cell.qw[q];
+}
+void oracle_fr_cell_lengths(void *h, double *len) {
+  auto *p = static_cast<FrHandle *>(h);
+  for (size_t c = 0; c < p->mesh.cells.size(); ++c) len[c] = p->mesh.cells[c].length;
+}
+int oracle_fr_residual(void *h, double const *u, double *r) {
+  return Guard([&] { static_cast<FrHandle *>(h)->scheme->Residual(u, r); });
+}
+int oracle_fr_rk3(void *h, double *u, double dt, int n_steps) {
+  return Guard([&] {
+    auto *p = static_cast<FrHandle *>(h);
+    size_t size = p->mesh.cells.size() * p->scheme->n * 5;
+    for (int s = 0; s < n_steps; ++s)
+      Rk3Step(size, u, dt, [&](double const *x, double *r) { p->scheme->Residual(x, r); });
+  });
+}
+double oracle_fr_time_step(void *h, double const *u, double cfl, double dt_guess) {
+  return static_cast<FrHandle *>(h)->scheme->TimeStep(u, cfl, dt_guess);
+}
+// Wall-clock seconds of `n` residual evaluations (CPU baseline timing).
+double oracle_fr_time_residuals(void *h, double const *u, double *r, int n) {
+  auto t0 = std::chrono::steady_clock::now();
+  for (int i = 0; i < n; ++i) static_cast<FrHandle *>(h)->scheme->Residual(u, r);
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // extern "C"

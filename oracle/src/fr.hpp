@@ -1,0 +1,276 @@
+// Oracle (test infrastructure): nodal flux reconstruction on hexahedra,
+// restating spatial/fr/general.hpp and spatial/fem.hpp.
+//
+// The reference's fr::General asserts that every face flux point is also a
+// solution point (Gauss-Lobatto nodes, fr/general.hpp:107-192).  On
+// Gauss-Legendre nodes the flux point lies at the end of the line of
+// solution points normal to the face; the natural generalisation used here
+// (and by the product) evaluates both the solution and the cached local flux
+// F~ = F J* through the line's Lagrange interpolant at that end point.  With
+// Lobatto nodes the interpolation weights are exactly (1, 0, ..., 0) and the
+// arithmetic reduces to fr/general.hpp:296-315 verbatim.
+#ifndef ORACLE_FR_HPP_
+#define ORACLE_FR_HPP_
+
+#include <vector>
+
+#include "core.hpp"
+#include "mesh.hpp"
+
+namespace oracle {
+
+enum BcKind { kBcSmart = 0, kBcInviscidWall = 1, kBcSupersonicOutlet = 2, kBcSupersonicInlet = 3 };
+struct BcEntry { int kind; Val5 state; };
+
+// Line (Q solution points normal to a face) of local face i_face through
+// the flux point with in-face node indices (a, b); returns the Q node ids.
+// The in-face index pairs follow fr/general.hpp:103-205:
+//   face 0 (zeta=-1): (i, j); 1 (eta=-1): (i, k); 2 (xi=+1): (j, k);
+//   face 3 (eta=+1): (i, k); 4 (xi=-1): (j, k); 5 (zeta=+1): (i, j).
+inline void FaceLine(int q, int i_face, int a, int b, int *ijk) {
+  for (int m = 0; m < q; ++m) {
+    switch (i_face) {
+    case 0: case 5: ijk[m] = (a * q + b) * q + m; break;
+    case 1: case 3: ijk[m] = (a * q + m) * q + b; break;
+    default: ijk[m] = (m * q + a) * q + b; break;
+    }
+  }
+}
+constexpr int kFaceDir[6] = { 2, 1, 0, 1, 0, 2 };        // xyz normal to the face
+constexpr double kFaceSign[6] = { -1, -1, +1, +1, -1, +1 };  // end of the line
+
+struct FrScheme {
+  Mesh const *mesh;
+  int q, n;  // 1-D points, nodes per cell
+  Lagrange1D line;
+  Vincent vincent;
+  int riemann;
+  std::vector<BcEntry> bcs;  // indexed by boundary tag
+  // per cell, per node: J* = det(J) J^{-1} and det(J) (fr/general.hpp:249-268)
+  std::vector<Mat3> jstar;
+  std::vector<double> det;
+  // per face point: (holder ijk, sharer ijk) of the node nearest the face,
+  // as fr/general.hpp:106-191 stores in FluxPointCache::ijk
+  std::vector<std::array<int, 4>> fp_index;  // holder (a, b), sharer (a, b)
+  std::vector<RotatedEuler> riemanns;        // [face][point], fem.hpp:95-115
+  double dmat[8][8];                         // dmat[i][m] = l_m'(x_i)
+  double lend[2][8];                         // l_m(-1), l_m(+1)
+  double gprime[2][8];                       // g_L'(x_m), g_R'(x_m)
+
+  FrScheme(Mesh const *m, double c_next, int riemann_kind, std::vector<BcEntry> bc)
+      : mesh(m), q(m->q), n(m->q * m->q * m->q), vincent(m->q - 1, c_next),
+        riemann(riemann_kind), bcs(std::move(bc)) {
+    line.q = q;
+    for (int i = 0; i < q; ++i) line.x[i] = mesh->x1d[i];
+    for (int i = 0; i < q; ++i) for (int m2 = 0; m2 < q; ++m2) dmat[i][m2] = line.Derivative(m2, line.x[i]);
+    for (int m2 = 0; m2 < q; ++m2) {
+      lend[0][m2] = line.Value(m2, -1.0);
+      lend[1][m2] = line.Value(m2, +1.0);
+      gprime[0][m2] = vincent.LeftDerivative(line.x[m2]);
+      gprime[1][m2] = vincent.RightDerivative(line.x[m2]);
+    }
+    size_t nc = mesh->cells.size();
+    jstar.resize(nc * n);
+    det.resize(nc * n);
+    for (size_t c = 0; c < nc; ++c) {
+      auto const &cell = mesh->cells[c];
+      for (int i = 0; i < q; ++i) for (int j = 0; j < q; ++j) for (int k = 0; k < q; ++k) {
+        int ijk = (i * q + j) * q + k;
+        Mat3 jac = cell.map.LocalToJacobian(line.x[i], line.x[j], line.x[k]);
+        Mat3 inv = inv3(jac);
+        double d = det3(jac);
+        for (int r = 0; r < 3; ++r) for (int s = 0; s < 3; ++s) jstar[c * n + ijk][r][s] = d * inv[r][s];
+        det[c * n + ijk] = cell.qw[ijk] / cell.qw_local[ijk];  // element.hpp:84-86
+      }
+    }
+    int nfp = q * q;
+    fp_index.resize(mesh->faces.size() * nfp);
+    riemanns.resize(mesh->faces.size() * nfp);
+    for (size_t f = 0; f < mesh->faces.size(); ++f) {
+      auto const &face = mesh->faces[f];
+      for (int p = 0; p < nfp; ++p) {
+        riemanns[f * nfp + p].kind = RiemannKind(riemann);
+        int a, b;
+        FindCollinear(mesh->cells[face.holder], face.holder_face, face.qp[p], &a, &b);
+        fp_index[f * nfp + p] = { a, b, -1, -1 };
+        if (face.sharer >= 0) {
+          FindCollinear(mesh->cells[face.sharer], face.sharer_face, add(face.qp[p], face.shift), &a, &b);
+          fp_index[f * nfp + p][2] = a;
+          fp_index[f * nfp + p][3] = b;
+        }
+      }
+    }
+  }
+
+  // polynomial/hexahedron.hpp:821-928 FindCollinearIndexByGlobal.
+  void FindCollinear(CellGeom const &cell, int i_face, Vec3 const &x, int *a, int *b) const {
+    double const *p = mesh->x1d;
+    for (int s = 0; s < q; ++s) for (int t = 0; t < q; ++t) {
+      Vec3 y;
+      switch (i_face) {
+      case 0: y = cell.map.LocalToGlobal(p[s], p[t], -1); break;
+      case 1: y = cell.map.LocalToGlobal(p[s], -1, p[t]); break;
+      case 2: y = cell.map.LocalToGlobal(+1, p[s], p[t]); break;
+      case 3: y = cell.map.LocalToGlobal(p[s], +1, p[t]); break;
+      case 4: y = cell.map.LocalToGlobal(-1, p[s], p[t]); break;
+      default: y = cell.map.LocalToGlobal(p[s], p[t], +1); break;
+      }
+      if (norm(sub(y, x)) < 1e-8) { *a = s; *b = t; return; }
+    }
+    throw std::runtime_error("FindCollinear: no match");
+  }
+
+  // The node nearest the face on the line, i.e. FluxPointCache::ijk.
+  int FluxPointNode(int i_face, int a, int b) const {
+    int ijk[8];
+    FaceLine(q, i_face, a, b, ijk);
+    return kFaceSign[i_face] < 0 ? ijk[0] : ijk[q - 1];
+  }
+
+  // J* at the flux point: local coordinates of the line's end.
+  Mat3 JstarAt(CellGeom const &cell, int i_face, int a, int b) const {
+    double const *p = mesh->x1d;
+    double e = kFaceSign[i_face];
+    double l[3];
+    switch (i_face) {
+    case 0: case 5: l[0] = p[a]; l[1] = p[b]; l[2] = e; break;
+    case 1: case 3: l[0] = p[a]; l[1] = e; l[2] = p[b]; break;
+    default: l[0] = e; l[1] = p[a]; l[2] = p[b]; break;
+    }
+    Mat3 jac = cell.map.LocalToJacobian(l[0], l[1], l[2]);
+    Mat3 inv = inv3(jac);
+    double d = det3(jac);
+    Mat3 r;
+    for (int s = 0; s < 3; ++s) for (int t = 0; t < 3; ++t) r[s][t] = d * inv[s][t];
+    return r;
+  }
+
+  // Column layout: u[(c * n + ijk) * 5 + k] (part.hpp:754-759, expansion.hpp:83-104).
+  void Residual(double const *u, double *r) {
+    size_t nc = mesh->cells.size();
+    std::vector<double> flux(nc * n * 15);  // F~ cache, fr/general.hpp:215
+    std::fill(r, r + nc * n * 5, 0.0);
+    // AddFluxDivergence, fr/general.hpp:271-288
+    for (size_t c = 0; c < nc; ++c) {
+      for (int ijk = 0; ijk < n; ++ijk) {
+        Val5 uq;
+        for (int k = 0; k < 5; ++k) uq[k] = u[(c * n + ijk) * 5 + k];
+        Flux53 f = GetFluxMatrix(uq);
+        Mat3 const &js = jstar[c * n + ijk];
+        for (int k = 0; k < 5; ++k) for (int d = 0; d < 3; ++d)
+          flux[((c * n + ijk) * 5 + k) * 3 + d] = f[k][0] * js[0][d] + f[k][1] * js[1][d] + f[k][2] * js[2][d];
+      }
+      for (int i = 0; i < q; ++i) for (int j = 0; j < q; ++j) for (int k = 0; k < q; ++k) {
+        int ijk = (i * q + j) * q + k;
+        for (int v = 0; v < 5; ++v) {
+          double val = 0;
+          for (int m = 0; m < q; ++m) {
+            val += flux[((c * n + (m * q + j) * q + k) * 5 + v) * 3 + 0] * dmat[i][m];
+            val += flux[((c * n + (i * q + m) * q + k) * 5 + v) * 3 + 1] * dmat[j][m];
+            val += flux[((c * n + (i * q + j) * q + m) * 5 + v) * 3 + 2] * dmat[k][m];
+          }
+          r[(c * n + ijk) * 5 + v] -= val;
+        }
+      }
+    }
+    int nfp = q * q;
+    auto trace = [&](int64_t c, int i_face, int a, int b, Val5 *uf, Val5 *fn) {
+      int ijk[8];
+      FaceLine(q, i_face, a, b, ijk);
+      double const *l = lend[kFaceSign[i_face] > 0];
+      int dir = kFaceDir[i_face];
+      uf->fill(0); fn->fill(0);
+      for (int m = 0; m < q; ++m)
+        for (int v = 0; v < 5; ++v) {
+          (*uf)[v] += l[m] * u[(c * n + ijk[m]) * 5 + v];
+          (*fn)[v] += l[m] * flux[((c * n + ijk[m]) * 5 + v) * 3 + dir];
+        }
+    };
+    auto correct = [&](int64_t c, int i_face, int a, int b, Val5 const &gap) {
+      int ijk[8];
+      FaceLine(q, i_face, a, b, ijk);
+      double const *g = gprime[kFaceSign[i_face] > 0];
+      for (int m = 0; m < q; ++m)
+        for (int v = 0; v < 5; ++v) r[(c * n + ijk[m]) * 5 + v] -= gap[v] * g[m];
+    };
+    auto face_scale = [&](int64_t c, int i_face, int a, int b) {
+      Mat3 js = JstarAt(mesh->cells[c], i_face, a, b);
+      int dir = kFaceDir[i_face];
+      Vec3 col{ js[0][dir], js[1][dir], js[2][dir] };
+      return kFaceSign[i_face] * norm(col);
+    };
+    // AddFluxOnLocalFaces -> AddFluxToHolderAndSharer, fr/general.hpp:409-434
+    for (int64_t f = 0; f < mesh->n_interior; ++f) {
+      auto const &face = mesh->faces[f];
+      for (int p = 0; p < nfp; ++p) {
+        auto const &ix = fp_index[f * nfp + p];
+        Val5 uh, fh, us, fs;
+        trace(face.holder, face.holder_face, ix[0], ix[1], &uh, &fh);
+        trace(face.sharer, face.sharer_face, ix[2], ix[3], &us, &fs);
+        Val5 fu = riemanns[f * nfp + p].GetFluxUpwind(face.frames[p], uh, us);
+        double sh = face_scale(face.holder, face.holder_face, ix[0], ix[1]);
+        double ss = face_scale(face.sharer, face.sharer_face, ix[2], ix[3]);
+        Val5 gh, gs;
+        for (int v = 0; v < 5; ++v) {
+          gh[v] = fu[v] * sh - fh[v];
+          gs[v] = fu[v] * (-ss) - fs[v];
+        }
+        correct(face.holder, face.holder_face, ix[0], ix[1], gh);
+        correct(face.sharer, face.sharer_face, ix[2], ix[3], gs);
+      }
+    }
+    // AddFluxOnBoundaries, fem.hpp:260-307 (family order), fr/general.hpp:435-640
+    const int family_order[] = { kBcSupersonicInlet, kBcSupersonicOutlet, kBcInviscidWall, kBcSmart };
+    for (int family : family_order) {
+      for (size_t tag = 0; tag < bcs.size(); ++tag) {
+        if (bcs[tag].kind != family) continue;
+        for (size_t f = mesh->n_interior; f < mesh->faces.size(); ++f) {
+          auto const &face = mesh->faces[f];
+          if (face.tag != (int)tag) continue;
+          for (int p = 0; p < nfp; ++p) {
+            auto const &ix = fp_index[f * nfp + p];
+            Val5 uh, fh;
+            trace(face.holder, face.holder_face, ix[0], ix[1], &uh, &fh);
+            auto &rs = riemanns[f * nfp + p];
+            Val5 fu;
+            switch (family) {
+            case kBcInviscidWall: fu = rs.GetFluxOnInviscidWall(face.frames[p], uh); break;
+            case kBcSupersonicOutlet: fu = rs.GetFluxOfState(face.frames[p], uh); break;
+            case kBcSupersonicInlet: fu = rs.GetFluxOfState(face.frames[p], bcs[tag].state); break;
+            default: fu = rs.GetFluxOnSmartBoundary(face.frames[p], uh, bcs[tag].state); break;
+            }
+            double sh = face_scale(face.holder, face.holder_face, ix[0], ix[1]);
+            Val5 gh;
+            for (int v = 0; v < 5; ++v) gh[v] = fu[v] * sh - fh[v];
+            correct(face.holder, face.holder_face, ix[0], ix[1], gh);
+          }
+        }
+      }
+    }
+    // divide by det(J) at each node, fr/general.hpp:249-268
+    for (size_t c = 0; c < nc; ++c)
+      for (int ijk = 0; ijk < n; ++ijk) {
+        double s = 1.0 / det[c * n + ijk];
+        for (int v = 0; v < 5; ++v) r[(c * n + ijk) * 5 + v] *= s;
+      }
+  }
+
+  // fem.hpp:417-435 with an explicit CFL number: every node is sampled.
+  double TimeStep(double const *u, double cfl, double dt_guess) const {
+    double min_dt = 1e100;
+    for (size_t c = 0; c < mesh->cells.size(); ++c) {
+      double max_speed = 1e-5;
+      for (int i = 0; i < n; ++i) {
+        Val5 uq;
+        for (int k = 0; k < 5; ++k) uq[k] = u[(c * n + i) * 5 + k];
+        max_speed = std::max(max_speed, GetMaximumSpeed(uq));
+      }
+      min_dt = std::min(min_dt, mesh->cells[c].length / max_speed);
+    }
+    return std::min(cfl * min_dt, dt_guess);
+  }
+};
+
+}  // namespace oracle
+
+#endif  // ORACLE_FR_HPP_

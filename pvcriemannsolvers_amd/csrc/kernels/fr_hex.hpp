@@ -1,0 +1,64 @@
+// Launch interface of the nodal FR kernels on affine hexahedra.
+#ifndef MINICFD_KERNELS_FR_HEX_HPP_
+#define MINICFD_KERNELS_FR_HEX_HPP_
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mcfd {
+
+constexpr int kMaxQ = 6;
+constexpr int kMaxBcs = 16;
+
+// 1-D tables of the solution-point line (Q points):
+//   d[i][m]   = l_m'(x_i)          (Lagrange derivative matrix)
+//   lend[e][m] = l_m(-1), l_m(+1)   (extrapolation to the line ends)
+//   gp[e][m]  = g_L'(x_m), g_R'(x_m) (Vincent correction derivatives)
+struct LineTables {
+  double d[kMaxQ][kMaxQ];
+  double lend[2][kMaxQ];
+  double gp[2][kMaxQ];
+};
+
+struct BcTable {
+  int32_t kind[kMaxBcs];
+  double state[kMaxBcs][5];
+};
+
+// Per-cell affine geometry record (20 doubles):
+//   [0..8]  J* = det(J) J^{-1} (row-major, F~ = F J*)
+//   [9]     1 / det(J)
+//   [10..15] signed face scale +-|J*.col(normal dir)| of local faces 0..5
+//   [16]    length = cbrt(volume) / 2
+constexpr int kGeoStride = 20;
+constexpr int kFrameStride = 6;  // n, t
+
+struct FrHexArgs {
+  const double *us;      // state of this stage, also read for neighbour traces
+  const double *u0;      // U0 for RK stages 2 and 3
+  double *out;           // residual (stage 0) or next state
+  const int32_t *nbr;    // [cell][6]: neighbour cell, or -(1 + boundary tag)
+  const int32_t *finfo;  // [cell][6]: frame id << 7 | holder << 6 | orient << 3 | neighbour face
+  const int32_t *geo_id; // [cell]
+  const double *geo;     // [geo][kGeoStride]
+  const double *frames;  // [frame][kFrameStride]
+  int64_t n_cells;
+  double dt;
+  int32_t stage;         // 0 residual, 1..3 SSP-RK3 stage
+  int32_t xcd_remap;     // remap block ids so each XCD sweeps a contiguous range
+};
+
+// Launches one residual / RK-stage pass for Q points per direction and the
+// given Riemann solver on `stream`.  Returns hipSuccess or the launch error.
+hipError_t LaunchFrHexStage(int q, int riemann, const FrHexArgs &args, const LineTables &lt,
+                            const BcTable &bc, hipStream_t stream);
+
+// Per-cell minimum of length / max wave speed over the solution points
+// (spatial/fem.hpp:417-435), reduced to one double in *dt_min (device).
+hipError_t LaunchFrHexTimeStep(int q, const double *u, const int32_t *geo_id, const double *geo,
+                               int64_t n_cells, unsigned long long *dt_min_bits, hipStream_t stream);
+
+}  // namespace mcfd
+
+#endif  // MINICFD_KERNELS_FR_HEX_HPP_

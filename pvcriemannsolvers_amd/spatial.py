@@ -1,0 +1,139 @@
+"""mini::spatial surface backed by libminicfd.
+
+``FiniteElement`` implements :class:`temporal.System` exactly like the
+reference's ``FiniteElement`` (spatial/fem.hpp:224-258): columns cross the
+boundary by value.  ``rk3_update`` is the fused device path of
+``RungeKutta<3>::Update`` (temporal/rk.hpp:31-49) that keeps the state in HBM.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _capi as C
+from .temporal import System
+
+
+class FiniteElement(System):
+    """A DG/FR spatial discretisation on one GPU (fem.hpp:47-436).
+
+    ``scheme``: "fr" (spatial/fr/general.hpp) or "dg" (spatial/dg/general.hpp).
+    ``riemann``: "exact", "hllc" or "roe".  ``bcs``: list of (kind, state) with
+    kind in {"smart", "inviscid_wall", "supersonic_outlet", "supersonic_inlet"}
+    indexed by the mesh's boundary tags (fem.hpp:196-222).
+    """
+
+    _RIEMANN = {"exact": C.RIEMANN_EXACT, "hllc": C.RIEMANN_HLLC, "roe": C.RIEMANN_ROE}
+    _BC = {"smart": C.BC_SMART, "inviscid_wall": C.BC_INVISCID_WALL,
+           "supersonic_outlet": C.BC_SUPERSONIC_OUTLET, "supersonic_inlet": C.BC_SUPERSONIC_INLET}
+
+    def __init__(self, mesh: dict, degree: int, scheme: str = "fr", riemann: str = "roe",
+                 nodes: str = "legendre", c_next: float = 1.0, bcs=(), precision: str = "fp64",
+                 host_only: bool = False):
+        lib = C.load()
+        self._keep = [np.ascontiguousarray(mesh["coords"], np.float64),
+                      np.ascontiguousarray(mesh["cells"], np.int64),
+                      np.ascontiguousarray(mesh["bface_nodes"], np.int64),
+                      np.ascontiguousarray(mesh["bface_tags"], np.int32)]
+        coords, cells, bn, bt = self._keep
+        md = C.MeshDesc()
+        md.cell_type = int(mesh.get("cell_type", C.HEXA_8))
+        md.n_nodes, md.coords = len(coords), C.ptr(coords, C._dp)
+        md.n_cells, md.cell_nodes = len(cells), C.ptr(cells, C._i64p)
+        md.n_bfaces, md.bface_nodes, md.bface_tags = len(bn), C.ptr(bn, C._i64p), C.ptr(bt, C._i32p)
+        for d in range(3):
+            md.period[d] = float(mesh["period"][d])
+        sd = C.SchemeDesc()
+        sd.scheme = C.SCHEME_FR_GENERAL if scheme == "fr" else C.SCHEME_DG_GENERAL
+        sd.degree = degree
+        sd.nodes = C.NODES_LOBATTO if nodes == "lobatto" else C.NODES_LEGENDRE
+        sd.riemann = self._RIEMANN[riemann]
+        sd.precision = C.PRECISION_MIXED if precision == "mixed" else C.PRECISION_FP64
+        sd.flags = C.FLAG_HOST_ONLY if host_only else 0
+        sd.c_next = c_next
+        bc_arr = (C.BcDesc * max(1, len(bcs)))()
+        for i, (kind, state) in enumerate(bcs):
+            bc_arr[i].kind = self._BC[kind]
+            for k in range(5):
+                bc_arr[i].state[k] = float(state[k]) if state is not None else 0.0
+        h = ctypes.c_void_p()
+        C.check(lib.mcfd_create(ctypes.byref(md), ctypes.byref(sd), bc_arr, len(bcs), ctypes.byref(h)))
+        self._h = h
+        n_cells, n_per_cell, size = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int64()
+        C.check(lib.mcfd_sizes(h, ctypes.byref(n_cells), ctypes.byref(n_per_cell), ctypes.byref(size)))
+        self.n_cells, self.n_per_cell, self.column_size = n_cells.value, n_per_cell.value, size.value
+        self.degree = degree
+
+    def close(self):
+        if getattr(self, "_h", None):
+            C.load().mcfd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    # ---- temporal::System
+    def set_time(self, t_curr):
+        C.check(C.load().mcfd_set_time(self._h, float(t_curr)))
+
+    def set_solution_column(self, column):
+        column = np.ascontiguousarray(column, np.float64).reshape(-1)
+        assert column.size == self.column_size
+        C.check(C.load().mcfd_set_solution_column(self._h, C.ptr(column, C._dp)))
+
+    def get_solution_column(self):
+        out = np.empty(self.column_size)
+        C.check(C.load().mcfd_get_solution_column(self._h, C.ptr(out, C._dp)))
+        return out
+
+    def get_residual_column(self):
+        out = np.empty(self.column_size)
+        C.check(C.load().mcfd_get_residual_column(self._h, C.ptr(out, C._dp)))
+        return out
+
+    # ---- fused device paths
+    def rk3_update(self, t_curr, dt):
+        C.check(C.load().mcfd_rk3_update(self._h, float(t_curr), float(dt)))
+
+    def rk3_stage(self, stage, t_curr, dt):
+        C.check(C.load().mcfd_rk3_stage(self._h, int(stage), float(t_curr), float(dt)))
+
+    def synchronize(self):
+        C.check(C.load().mcfd_synchronize(self._h))
+
+    def get_time_step(self, cfl, dt_guess=1e300):
+        dt = ctypes.c_double()
+        C.check(C.load().mcfd_get_time_step(self._h, float(cfl), float(dt_guess), ctypes.byref(dt)))
+        return dt.value
+
+    def time_rk3(self, dt, n_steps):
+        total, kern = ctypes.c_double(), ctypes.c_double()
+        C.check(C.load().mcfd_time_rk3(self._h, float(dt), int(n_steps), ctypes.byref(total),
+                                       ctypes.byref(kern)))
+        return total.value, kern.value
+
+    # ---- set-up helpers
+    def sample_points(self):
+        n = ctypes.c_int32()
+        C.check(C.load().mcfd_sample_points(self._h, ctypes.byref(n), None))
+        out = np.empty((self.n_cells, n.value, 3))
+        C.check(C.load().mcfd_sample_points(self._h, ctypes.byref(n), C.ptr(out, C._dp)))
+        return out
+
+    def approximate(self, values):
+        values = np.ascontiguousarray(values, np.float64)
+        C.check(C.load().mcfd_approximate(self._h, C.ptr(values, C._dp)))
+
+    def faces(self):
+        nf, ni = ctypes.c_int64(), ctypes.c_int64()
+        C.check(C.load().mcfd_face_count(self._h, ctypes.byref(nf), ctypes.byref(ni)))
+        nf = nf.value
+        nfp = (self.degree + 1) ** 2
+        holder, sharer = np.empty(nf, np.int64), np.empty(nf, np.int64)
+        hf, sf = np.empty(nf, np.int32), np.empty(nf, np.int32)
+        ijk = np.empty((nf, nfp, 2), np.int32)
+        C.check(C.load().mcfd_faces(self._h, C.ptr(holder, C._i64p), C.ptr(sharer, C._i64p),
+                                    C.ptr(hf, C._i32p), C.ptr(sf, C._i32p), C.ptr(ijk, C._i32p)))
+        return {"n_interior": ni.value, "holder": holder, "sharer": sharer, "holder_face": hf,
+                "sharer_face": sf, "fp_ijk": ijk}

@@ -1,0 +1,206 @@
+"""ctypes loader of the oracle (oracle/_build/liboracle.so): test infrastructure.
+
+Only tests, the smoke check and bench.py's CPU-baseline leg import this.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_lib = None
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True,
+                       stdout=subprocess.DEVNULL)
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.oracle_last_error.restype = ctypes.c_char_p
+    lib.oracle_fr_create.argtypes = [ctypes.c_int64, _dp, ctypes.c_int64, _i64p, ctypes.c_int64, _i64p,
+                                     _i32p, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                     ctypes.c_int, _i32p, _dp, ctypes.POINTER(ctypes.c_void_p)]
+    lib.oracle_fr_n_faces.restype = ctypes.c_int64
+    lib.oracle_fr_n_faces.argtypes = [ctypes.c_void_p, _i64p]
+    lib.oracle_fr_faces.argtypes = [ctypes.c_void_p, _i64p, _i64p, _i32p, _i32p, _i32p, _dp]
+    lib.oracle_fr_node_coords.argtypes = [ctypes.c_void_p, _dp]
+    lib.oracle_fr_node_weights.argtypes = [ctypes.c_void_p, _dp]
+    lib.oracle_fr_cell_lengths.argtypes = [ctypes.c_void_p, _dp]
+    lib.oracle_fr_residual.argtypes = [ctypes.c_void_p, _dp, _dp]
+    lib.oracle_fr_rk3.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, ctypes.c_int]
+    lib.oracle_fr_time_step.restype = ctypes.c_double
+    lib.oracle_fr_time_step.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, ctypes.c_double]
+    lib.oracle_fr_time_residuals.restype = ctypes.c_double
+    lib.oracle_fr_time_residuals.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
+    lib.oracle_fr_destroy.argtypes = [ctypes.c_void_p]
+    lib.oracle_quadrature.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
+    lib.oracle_vincent.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp]
+    lib.oracle_lagrange.argtypes = [ctypes.c_int, _dp, ctypes.c_double, _dp, _dp]
+    for name in ("oracle_conservative_to_primitive", "oracle_primitive_to_conservative",
+                 "oracle_flux_matrix"):
+        getattr(lib, name).argtypes = [_dp, _dp]
+    lib.oracle_exact_primitive.argtypes = [_dp, _dp, _dp]
+    lib.oracle_riemann_flux.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
+    lib.oracle_wall_flux.argtypes = [_dp, _dp, _dp]
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(load().oracle_last_error().decode())
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def quadrature(lobatto, q):
+    x, w = np.zeros(q), np.zeros(q)
+    _check(load().oracle_quadrature(int(lobatto), q, _ptr(x, _dp), _ptr(w, _dp)))
+    return x, w
+
+
+def vincent(degree, c_next, s):
+    out = np.zeros(4)
+    load().oracle_vincent(degree, c_next, s, _ptr(out, _dp))
+    return out
+
+
+def lagrange(x, s):
+    x = f64(x)
+    v, d = np.zeros(len(x)), np.zeros(len(x))
+    load().oracle_lagrange(len(x), _ptr(x, _dp), s, _ptr(v, _dp), _ptr(d, _dp))
+    return v, d
+
+
+def conservative_to_primitive(cv):
+    cv, pv = f64(cv), np.zeros(5)
+    load().oracle_conservative_to_primitive(_ptr(cv, _dp), _ptr(pv, _dp))
+    return pv
+
+
+def primitive_to_conservative(pv):
+    pv, cv = f64(pv), np.zeros(5)
+    load().oracle_primitive_to_conservative(_ptr(pv, _dp), _ptr(cv, _dp))
+    return cv
+
+
+def flux_matrix(cv):
+    cv, f = f64(cv), np.zeros(15)
+    load().oracle_flux_matrix(_ptr(cv, _dp), _ptr(f, _dp))
+    return f.reshape(5, 3)
+
+
+def exact_primitive(pl, pr):
+    pl, pr, out = f64(pl), f64(pr), np.zeros(5)
+    load().oracle_exact_primitive(_ptr(pl, _dp), _ptr(pr, _dp), _ptr(out, _dp))
+    return out
+
+
+def riemann_flux(kind, frame, cl, cr):
+    frame, cl, cr, out = f64(frame).reshape(9), f64(cl), f64(cr), np.zeros(5)
+    load().oracle_riemann_flux(kind, _ptr(frame, _dp), _ptr(cl, _dp), _ptr(cr, _dp), _ptr(out, _dp))
+    return out
+
+
+def wall_flux(frame, c):
+    frame, c, out = f64(frame).reshape(9), f64(c), np.zeros(5)
+    load().oracle_wall_flux(_ptr(frame, _dp), _ptr(c, _dp), _ptr(out, _dp))
+    return out
+
+
+class FrOracle:
+    """Nodal FR on hexahedra (spatial/fr/general.hpp restated)."""
+
+    def __init__(self, mesh: dict, q: int, riemann: int, lobatto=False, c_next=1.0, bcs=()):
+        lib = load()
+        self.mesh = mesh
+        self.q = q
+        self.n = q ** 3
+        self.n_cells = len(mesh["cells"])
+        self._keep = [f64(mesh["coords"]), np.ascontiguousarray(mesh["cells"], np.int64),
+                      np.ascontiguousarray(mesh["bface_nodes"], np.int64),
+                      np.ascontiguousarray(mesh["bface_tags"], np.int32), f64(mesh["period"])]
+        kinds = np.array([b[0] for b in bcs] or [0], dtype=np.int32)
+        states = f64(np.array([b[1] for b in bcs] or [[0.0] * 5]).reshape(-1))
+        self._keep += [kinds, states]
+        h = ctypes.c_void_p()
+        c, cells, bn, bt, per = self._keep[:5]
+        _check(lib.oracle_fr_create(len(c), _ptr(c, _dp), len(cells), _ptr(cells, _i64p), len(bn),
+                                    _ptr(bn, _i64p), _ptr(bt, _i32p), _ptr(per, _dp), q, int(lobatto),
+                                    float(c_next), int(riemann), len(bcs), _ptr(kinds, _i32p),
+                                    _ptr(states, _dp), ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().oracle_fr_destroy(self.h)
+            self.h = None
+
+    @property
+    def size(self):
+        return self.n_cells * self.n * 5
+
+    def node_coords(self):
+        out = np.zeros((self.n_cells, self.n, 3))
+        load().oracle_fr_node_coords(self.h, _ptr(out, _dp))
+        return out
+
+    def node_weights(self):
+        out = np.zeros((self.n_cells, self.n))
+        load().oracle_fr_node_weights(self.h, _ptr(out, _dp))
+        return out
+
+    def cell_lengths(self):
+        out = np.zeros(self.n_cells)
+        load().oracle_fr_cell_lengths(self.h, _ptr(out, _dp))
+        return out
+
+    def faces(self):
+        n_int = ctypes.c_int64()
+        nf = load().oracle_fr_n_faces(self.h, ctypes.byref(n_int))
+        nfp = self.q * self.q
+        holder, sharer = np.zeros(nf, np.int64), np.zeros(nf, np.int64)
+        hf, sf = np.zeros(nf, np.int32), np.zeros(nf, np.int32)
+        ijk = np.zeros((nf, nfp, 2), np.int32)
+        frames = np.zeros((nf, nfp, 3, 3))
+        load().oracle_fr_faces(self.h, _ptr(holder, _i64p), _ptr(sharer, _i64p), _ptr(hf, _i32p),
+                               _ptr(sf, _i32p), _ptr(ijk, _i32p), _ptr(frames, _dp))
+        return {"n_interior": n_int.value, "holder": holder, "sharer": sharer, "holder_face": hf,
+                "sharer_face": sf, "fp_ijk": ijk, "frames": frames}
+
+    def residual(self, u):
+        u = f64(u)
+        r = np.zeros_like(u)
+        _check(load().oracle_fr_residual(self.h, _ptr(u, _dp), _ptr(r, _dp)))
+        return r
+
+    def rk3(self, u, dt, n_steps=1):
+        u = f64(u).copy()
+        _check(load().oracle_fr_rk3(self.h, _ptr(u, _dp), float(dt), int(n_steps)))
+        return u
+
+    def time_step(self, u, cfl, dt_guess=1e300):
+        u = f64(u)
+        return load().oracle_fr_time_step(self.h, _ptr(u, _dp), cfl, dt_guess)
+
+    def time_residuals(self, u, n):
+        u = f64(u)
+        r = np.zeros_like(u)
+        return load().oracle_fr_time_residuals(self.h, _ptr(u, _dp), _ptr(r, _dp), n)

@@ -1,0 +1,150 @@
+"""Parity of the HIP FR path (libminicfd) with the oracle, through the C ABI.
+
+Tolerances: the oracle and the kernels evaluate the same FP64 formulas in a
+different summation order (sum-factorised vs dense contractions, per-face vs
+per-node accumulation), so residuals agree to <= 1e-12 relative to the
+residual's max norm, and states after N RK3 steps to <= 1e-12 relative.
+
+The exact Riemann solver is the exception: the reference's FindRoot
+(riemann/euler/exact.hpp:140-152) stops as soon as f(p) >= -1e-8, so p* is
+only defined to about 1e-8; a last-bit difference in a face trace can add or
+drop one Newton step.  Its residual tolerance is therefore 1e-7 relative.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from pvcriemannsolvers_amd import meshgen, spatial
+
+pytestmark = pytest.mark.gpu
+
+GAMMA = 1.4
+
+
+def conservative(rho, vel, p):
+    rho = np.asarray(rho, dtype=np.float64)
+    u = np.zeros(rho.shape + (5,))
+    u[..., 0] = rho
+    for d in range(3):
+        u[..., 1 + d] = rho * vel[d]
+    u[..., 4] = p / (GAMMA - 1) + 0.5 * rho * (vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2)
+    return u
+
+
+def density_wave(x, seed=1, amp=0.2):
+    rng = np.random.default_rng(seed)
+    rho = 1 + amp * np.sin(2 * np.pi * (x[..., 0] + x[..., 1] + x[..., 2]))
+    for _ in range(4):
+        k = rng.integers(-2, 3, size=3)
+        rho = rho + rng.uniform(-1e-3, 1e-3) * np.sin(2 * np.pi * (x @ k) + rng.uniform(0, 2 * np.pi))
+    return conservative(rho, (1.0, 0.5, 0.25), 1.0)
+
+
+def sod_like(x, seed=2):
+    rng = np.random.default_rng(seed)
+    left = conservative(1.0, (0.1, 0.0, 0.0), 1.0)
+    right = conservative(0.125, (0.0, 0.0, 0.0), 0.1)
+    u = np.where((x[..., 0] < 0.5)[..., None], left, right)
+    return u * (1 + 1e-3 * rng.standard_normal(u.shape[:-1]))[..., None]
+
+
+RIEMANN = {"exact": 0, "hllc": 1, "roe": 2}
+BC_KIND = {"smart": 0, "inviscid_wall": 1, "supersonic_outlet": 2, "supersonic_inlet": 3}
+
+
+def make_pair(mesh, degree, riemann, bcs=(), nodes="legendre", c_next=1.0):
+    o = ol.FrOracle(mesh, q=degree + 1, riemann=RIEMANN[riemann], lobatto=(nodes == "lobatto"),
+                    c_next=c_next, bcs=[(BC_KIND[k], s if s is not None else [0] * 5) for k, s in bcs])
+    g = spatial.FiniteElement(mesh, degree=degree, riemann=riemann, nodes=nodes, c_next=c_next, bcs=bcs)
+    return o, g
+
+
+TOL = {"roe": 1e-12, "hllc": 1e-12, "exact": 1e-7}
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3, 4])
+@pytest.mark.parametrize("riemann", ["roe", "hllc", "exact"])
+def test_periodic_residual(degree, riemann):
+    mesh = meshgen.box_hex(4, 3, 5, periodic=(True, True, True), shuffle_seed=degree)
+    o, g = make_pair(mesh, degree, riemann)
+    u = density_wave(o.node_coords())
+    g.set_solution_column(u.reshape(-1))
+    assert rel(g.get_residual_column(), o.residual(u).reshape(-1)) < TOL[riemann]
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3])
+@pytest.mark.parametrize("riemann", ["hllc", "exact", "roe"])
+def test_bounded_box_residual(degree, riemann):
+    mesh = meshgen.box_hex(6, 2, 3, hi=(1.0, 0.1, 0.1), shuffle_seed=11)
+    left, right = conservative(1.0, (0.1, 0, 0), 1.0), conservative(0.125, (0, 0, 0), 0.1)
+    bcs = [("smart", left), ("smart", right)] + [("inviscid_wall", None)] * 4
+    o, g = make_pair(mesh, degree, riemann, bcs)
+    u = sod_like(o.node_coords())
+    g.set_solution_column(u.reshape(-1))
+    assert rel(g.get_residual_column(), o.residual(u).reshape(-1)) < TOL[riemann]
+
+
+def test_supersonic_boundaries_and_lobatto():
+    mesh = meshgen.box_hex(5, 2, 2, hi=(1.0, 0.2, 0.2))
+    inflow = conservative(1.0, (3.0, 0.1, 0), 1.0)
+    bcs = [("supersonic_inlet", inflow), ("supersonic_outlet", None)] + [("inviscid_wall", None)] * 4
+    o, g = make_pair(mesh, 2, "hllc", bcs, nodes="lobatto", c_next=2 / 5)
+    x = o.node_coords()
+    u = conservative(1 + 0.1 * np.sin(2 * np.pi * x[..., 0]), (3.0, 0.1, 0.0), 1.0)
+    g.set_solution_column(u.reshape(-1))
+    assert rel(g.get_residual_column(), o.residual(u).reshape(-1)) < 1e-12
+
+
+@pytest.mark.parametrize("degree", [2, 3])
+def test_rk3_steps(degree):
+    mesh = meshgen.box_hex(4, 4, 4, periodic=(True, True, True), shuffle_seed=5)
+    o, g = make_pair(mesh, degree, "roe")
+    u = density_wave(o.node_coords(), seed=3)
+    dt = o.time_step(u.reshape(-1), 0.1)
+    assert abs(g_dt := (g.set_solution_column(u.reshape(-1)) or g.get_time_step(0.1))) > 0
+    assert abs(g_dt - dt) <= 1e-14 * dt
+    ref = o.rk3(u.reshape(-1), dt, n_steps=5)
+    for s in range(5):
+        g.rk3_update(s * dt, dt)
+    assert rel(g.get_solution_column(), ref) < 1e-12
+
+
+def test_system_contract_matches_fused_path():
+    """RungeKutta(3) through the System methods == the fused device update."""
+    from pvcriemannsolvers_amd import temporal
+    mesh = meshgen.box_hex(3, 3, 3, periodic=(True, True, True))
+    o, g = make_pair(mesh, 2, "roe")
+    u = density_wave(o.node_coords(), seed=4).reshape(-1)
+    dt = 1e-3
+    g.set_solution_column(u)
+    temporal.RungeKutta(3).update(g, 0.0, dt)
+    via_system = g.get_solution_column()
+    g.set_solution_column(u)
+    g.rk3_update(0.0, dt)
+    fused = g.get_solution_column()
+    assert rel(fused, via_system) < 1e-14
+    assert rel(fused, o.rk3(u, dt)) < 1e-12
+
+
+def test_free_stream_and_conservation_64cubed():
+    """Full-size config (64^3, p=3, Roe, periodic): size-independent properties."""
+    mesh = meshgen.box_hex(64, 64, 64, periodic=(True, True, True))
+    g = spatial.FiniteElement(mesh, degree=3, riemann="roe")
+    x = g.sample_points()
+    u = conservative(np.ones(x.shape[:-1]), (1.0, 0.5, 0.25), 1.0)
+    g.set_solution_column(u.reshape(-1))
+    assert np.abs(g.get_residual_column()).max() < 1e-9
+    u = density_wave(x)
+    g.set_solution_column(u.reshape(-1))
+    r = g.get_residual_column().reshape(x.shape[0], -1, 5)
+    q = 4
+    w1 = np.array([(18 - 30 ** 0.5) / 36, (18 + 30 ** 0.5) / 36, (18 + 30 ** 0.5) / 36, (18 - 30 ** 0.5) / 36])
+    w = np.einsum("i,j,k->ijk", w1, w1, w1).reshape(-1)
+    total = np.einsum("cnk,n->k", r, w)
+    assert np.abs(total).max() < 1e-9 * np.abs(r).max() * r.shape[0]
+    # deterministic: a second evaluation is bitwise identical
+    assert np.array_equal(r.reshape(-1), g.get_residual_column())
