@@ -1,0 +1,162 @@
+"""Benchmark: RK-stage DOF-updates/s of the FR residual + SSP-RK3 on one MI355X.
+
+Workload (the headline configuration): nodal FR with the DG correction
+function on Gauss-Legendre points, degree p = 3 (4^3 solution points per
+cell), Roe flux, SSP-RK3, on a periodic 64^3 hexahedral box [0,1]^3
+(262 144 cells, 16.8 M DOFs x 5 variables), FP64 throughout.  Initial state:
+rho = 1 + 0.2 sin(2 pi (x+y+z)) + 8 seeded Fourier modes with |k| <= 4 and
+amplitude <= 1e-3, u = (1, 0.5, 0.25), p = 1; dt from CFL 0.1.
+
+A "step" is one SSP-RK3 time step = 3 fused residual+stage launches.
+value = 3 * n_cells * N / (seconds per step)   [RK-stage DOF-updates/s]
+
+The roofline object prices the stage kernel against HBM with the algorithmic
+byte count B_stage = n_cells * (5 * N * 8 * m + 4 * 6): m = 2 for stage 1
+(read U0, write U1), 3 for stages 2 and 3 (read U_s, read U0, write U_next),
+plus one 4-byte neighbour index per face.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+GAMMA = 1.4
+
+
+def initial_state(x: np.ndarray, seed: int = 2024) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    s = x[..., 0] + x[..., 1] + x[..., 2]
+    rho = 1 + 0.2 * np.sin(2 * np.pi * s)
+    modes = 0
+    while modes < 8:
+        k = rng.integers(-4, 5, size=3)
+        if not (0 < np.linalg.norm(k) <= 4):
+            continue
+        rho += rng.uniform(-1e-3, 1e-3) * np.sin(2 * np.pi * (x @ k.astype(np.float64)) + rng.uniform(0, 2 * np.pi))
+        modes += 1
+    vel = (1.0, 0.5, 0.25)
+    u = np.empty(rho.shape + (5,))
+    u[..., 0] = rho
+    for d in range(3):
+        u[..., 1 + d] = rho * vel[d]
+    u[..., 4] = 1.0 / (GAMMA - 1) + 0.5 * rho * sum(v * v for v in vel)
+    return u
+
+
+def stage_bytes(n_cells: int, n_per_cell: int) -> tuple[float, float]:
+    per_cell = lambda m: 5 * n_per_cell * 8 * m + 4 * 6
+    return n_cells * per_cell(2), n_cells * per_cell(3)
+
+
+def read_traffic(name: str):
+    """HBM bytes per stage launch from the committed rocprofv3 PMC summary."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(n: int, steps: int, degree: int) -> dict:
+    """The oracle (test infrastructure) timed on the host: 1 thread."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    from pvcriemannsolvers_amd import meshgen
+    mesh = meshgen.box_hex(n, n, n, periodic=(True, True, True))
+    o = ol.FrOracle(mesh, q=degree + 1, riemann=2)
+    u = initial_state(o.node_coords()).reshape(-1)
+    dt = o.time_step(u, 0.1)
+    t0 = time.perf_counter()
+    o.rk3(u, dt, n_steps=steps)
+    sec = time.perf_counter() - t0
+    dofs = n ** 3 * (degree + 1) ** 3
+    return {"value": 3 * steps * dofs / sec, "unit": "DOF-updates/s", "cores": 1, "kind": "port",
+            "sample": f"oracle FR p={degree} Roe SSP-RK3, periodic {n}^3 box ({n ** 3} cells), "
+                      f"{steps} steps = {3 * steps} stages in {sec:.1f} s, single thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=64, help="cells per direction")
+    ap.add_argument("--degree", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--cpu-n", type=int, default=32)
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(sharding is described in DESIGN.md); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from pvcriemannsolvers_amd import meshgen, spatial
+
+    n, p = args.n, args.degree
+    mesh = meshgen.box_hex(n, n, n, periodic=(True, True, True))
+    g = spatial.FiniteElement(mesh, degree=p, riemann="roe")
+    x = g.sample_points()
+    u0 = initial_state(x)
+    del x
+    g.set_solution_column(u0.reshape(-1))
+    del u0
+    dt = g.get_time_step(0.1)
+    print(f"[bench] {n}^3 cells, p={p}, dt={dt:.4e}; warmup {args.warmup} steps", file=sys.stderr, flush=True)
+    for s in range(args.warmup):
+        g.rk3_update(s * dt, dt)
+    g.synchronize()
+    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    total_ms, kernel_ms = g.time_rk3(dt, args.steps)
+    g.synchronize()
+    wall = time.perf_counter() - t0
+    state = g.get_solution_column()
+    if not np.isfinite(state).all():
+        raise RuntimeError("non-finite state after the timed steps")
+
+    n_cells, npc = g.n_cells, g.n_per_cell
+    ms_per_step = wall * 1e3 / args.steps
+    value = 3 * n_cells * npc / (ms_per_step * 1e-3)
+    b1, b23 = stage_bytes(n_cells, npc)
+    avg_bytes = (b1 + 2 * b23) / 3
+    avg_kernel_s = kernel_ms * 1e-3 / (3 * args.steps)
+    achieved = avg_bytes / avg_kernel_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": read_traffic(f"fr_hex_stage_{n}cubed_p{p}_hbm_traffic.json"),
+                "kernel": "FrHexStageKernel<%d,2>" % (p + 1),
+                "avg_kernel_ms": round(avg_kernel_s * 1e3, 4),
+                "algorithmic_bytes_per_launch": int(avg_bytes)}
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.cpu_n, args.cpu_steps, p)
+    line = {
+        "metric": "RK-stage DOF-updates/s, mixed vs FP64 at stated L2 error; % HBM roofline",
+        "value": value, "unit": "DOF-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"FR p={p} Gauss-Legendre, Roe, SSP-RK3, periodic {n}^3 hex box",
+                   "cells": n_cells, "nodes_per_cell": npc, "dofs_per_variable": n_cells * npc,
+                   "precision": "fp64", "rk_stages_per_step": 3, "dt": dt,
+                   "event_ms_per_step": total_ms / args.steps},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -94,6 +94,7 @@ struct mcfd_solver {
   int64_t n_cells = 0;
   double t = 0;
   bool host_only = false;
+  bool diag_metric = false;  // every geometry record has a diagonal J*
   hipStream_t stream = nullptr;
   mcfd::HexConnectivity conn;
   mcfd::LineTables lt;
@@ -116,6 +117,7 @@ struct mcfd_solver {
     a.dt = dt;
     a.stage = stage;
     a.xcd_remap = 1;
+    a.diag_metric = diag_metric ? 1 : 0;
     switch (stage) {
     case 0: a.us = u0.ptr; a.out = res.ptr; break;
     case 1: a.us = u0.ptr; a.out = u1.ptr; break;
@@ -205,6 +207,11 @@ int mcfd_create(const mcfd_mesh_desc *mesh, const mcfd_scheme_desc *scheme, cons
       int64_t n0 = mesh->cell_nodes[8 * c];
       h->corner0[c] = { mesh->coords[3 * n0], mesh->coords[3 * n0 + 1], mesh->coords[3 * n0 + 2] };
     }
+
+    h->diag_metric = true;
+    for (size_t r = 0; r < geo.size(); r += mcfd::kGeoStride)
+      for (int e = 0; e < 9; ++e)
+        if (e % 4 != 0 && geo[r + e] != 0.0) h->diag_metric = false;
 
     // per-cell face records
     std::map<std::array<double, mcfd::kFrameStride>, int32_t> frame_ids;

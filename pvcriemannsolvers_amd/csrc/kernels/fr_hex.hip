@@ -28,18 +28,30 @@ namespace {
 
 template <int Q>
 struct Shape {
-  static constexpr int kPoints = Q * Q * Q;
-  static constexpr int kFacePoints = Q * Q;
-  static constexpr int kCells = 256 / kPoints;  // cells per workgroup
+  static constexpr int kPoints = Q * Q * Q;        // solution points per cell
+  static constexpr int kFacePoints = Q * Q;        // flux points per face
+  static constexpr int kFaceTasks = 6 * Q * Q;     // flux points per cell
+  static constexpr int kCells = 256 / kPoints;     // cells per workgroup
+  static constexpr int kPad = kPoints + 1;         // odd SoA stride (LDS banks)
+  static constexpr int kFacePad = kFaceTasks + 1;
+  static constexpr int kScratch = 5 * (kPad > kFacePad ? kPad : kFacePad);
+  static constexpr int kTasks = (kCells * kFaceTasks + 255) / 256;  // per thread
   static_assert(kCells >= 1, "at most 256 solution points per cell");
 };
 
-__device__ __forceinline__ int LineNode(int q, int f, int a, int b, int m) {
-  switch (f) {
-  case 0: case 5: return (a * q + b) * q + m;
-  case 1: case 3: return (a * q + m) * q + b;
-  default: return (m * q + a) * q + b;
-  }
+// Line of solution points through in-face point (a, b) of local face f:
+// node(m) = a * A + b * B + m * S.  Branch-free for lane-varying f.
+struct LineIndex { int a_mul, b_mul, stride, side, dir; };
+__device__ __forceinline__ LineIndex FaceLineIndex(int q, int f) {
+  const int dir = (2118 >> (2 * f)) & 3;  // {2, 1, 0, 1, 0, 2}
+  const int side = (0x2C >> f) & 1;       // {0, 0, 1, 1, 0, 1}
+  LineIndex li;
+  li.dir = dir;
+  li.side = side;
+  li.a_mul = dir == 0 ? q : q * q;
+  li.b_mul = dir == 2 ? q : 1;
+  li.stride = dir == 0 ? q * q : (dir == 1 ? q : 1);
+  return li;
 }
 
 // Extrapolation to a line end: a fixed-order fused chain so that both cells of
@@ -62,13 +74,54 @@ __device__ __forceinline__ int64_t LogicalBlock(int32_t remap) {
   return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
 }
 
-template <int Q, int kRiemann>
+template <int kRiemann>
+__device__ __forceinline__ State FaceFlux(FaceFrame const &f, State const &cl, State const &cr) {
+  if constexpr (kRiemann == kRoe) {
+    State l = FastPrimitive(cl), r = FastPrimitive(cr);
+    GlobalToNormal(f, &l);
+    GlobalToNormal(f, &r);
+    State flux = FastRoeFlux(l, r);
+    NormalToGlobal(f, &flux);
+    return flux;
+  } else {
+    return UpwindFlux<kRiemann>(f, cl, cr);
+  }
+}
+
+// Column d of the local flux F~ = F J* at one solution point.
+template <bool kDiag>
+__device__ __forceinline__ void LocalFluxColumn(State const &c, State const &p, const double *js, int d,
+                                                double *out) {
+  const double rh = c.v[4] + p.v[4];
+  if constexpr (kDiag) {
+    const double vel = p.v[1 + d], s = js[4 * d];
+    out[0] = c.v[1 + d] * s;
+    out[1] = (c.v[1] * vel + (d == 0 ? p.v[4] : 0.0)) * s;
+    out[2] = (c.v[2] * vel + (d == 1 ? p.v[4] : 0.0)) * s;
+    out[3] = (c.v[3] * vel + (d == 2 ? p.v[4] : 0.0)) * s;
+    out[4] = rh * vel * s;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) out[k] = 0;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      const double vel = p.v[1 + e], s = js[3 * e + d];
+      out[0] += c.v[1 + e] * s;
+      out[1] += (c.v[1] * vel + (e == 0 ? p.v[4] : 0.0)) * s;
+      out[2] += (c.v[2] * vel + (e == 1 ? p.v[4] : 0.0)) * s;
+      out[3] += (c.v[3] * vel + (e == 2 ? p.v[4] : 0.0)) * s;
+      out[4] += rh * vel * s;
+    }
+  }
+}
+
+template <int Q, int kRiemann, bool kDiag>
 __global__ __launch_bounds__(256) void FrHexStageKernel(FrHexArgs a, LineTables lt, BcTable bc) {
   using S = Shape<Q>;
-  constexpr int NP = S::kPoints, NFP = S::kFacePoints, C = S::kCells;
-  __shared__ double s_u[C * NP * 5];
-  __shared__ double s_f[C * NP * 15];
-  __shared__ double s_gap[C * 6 * NFP * 5];
+  constexpr int NP = S::kPoints, NFP = S::kFacePoints, NT = S::kFaceTasks, C = S::kCells;
+  constexpr int PAD = S::kPad, FPAD = S::kFacePad, T = S::kTasks;
+  __shared__ double s_u[C * 5 * PAD];         // states, SoA per cell
+  __shared__ double s_x[C * S::kScratch];     // F~ column per pass, then flux gaps
 
   const int tid = threadIdx.x;
   const int64_t c0 = LogicalBlock(a.xcd_remap) * C;
@@ -76,51 +129,101 @@ __global__ __launch_bounds__(256) void FrHexStageKernel(FrHexArgs a, LineTables 
   const int nc = (int)min<int64_t>(C, a.n_cells - c0);
   const int64_t base = c0 * NP * 5;
 
-  // 0. flat load of the cells' states
-  for (int e = tid; e < nc * NP * 5; e += 256) s_u[e] = a.us[base + e];
-  __syncthreads();
-
-  // 1. local flux F~ = F J* at every solution point
-  for (int w = tid; w < nc * NP; w += 256) {
-    const int cell = w / NP;
-    const double *g = a.geo + (int64_t)a.geo_id[c0 + cell] * kGeoStride;
-    State u;
+  // ---- node phase: load, primitive, SoA copy of the state
+  const bool node_active = tid < nc * NP;
+  const int ncell = tid / NP, node = tid - (tid / NP) * NP;
+  const int ni = node / (Q * Q), nj = (node / Q) % Q, nk = node % Q;
+  State cu, pu;
+  const double *js = nullptr;
+  double inv_det = 0;
+  if (node_active) {
 #pragma unroll
-    for (int k = 0; k < 5; ++k) u.v[k] = s_u[w * 5 + k];
-    double f[5][3];
-    FluxMatrix(u, f);
+    for (int k = 0; k < 5; ++k) cu.v[k] = a.us[base + tid * 5 + k];
+    pu = FastPrimitive(cu);
+    js = a.geo + (int64_t)a.geo_id[c0 + ncell] * kGeoStride;
+    inv_det = js[9];
 #pragma unroll
-    for (int k = 0; k < 5; ++k)
-#pragma unroll
-      for (int d = 0; d < 3; ++d)
-        s_f[w * 15 + k * 3 + d] = f[k][0] * g[d] + f[k][1] * g[3 + d] + f[k][2] * g[6 + d];
+    for (int k = 0; k < 5; ++k) s_u[(ncell * 5 + k) * PAD + node] = cu.v[k];
   }
-  __syncthreads();
 
-  // 2. face points: traces, upwind flux, flux gap
-  for (int w = tid; w < nc * 6 * NFP; w += 256) {
-    const int cell = w / (6 * NFP);
-    const int rem = w - cell * 6 * NFP;
-    const int f = rem / NFP, fp = rem - f * NFP;
-    const int ia = fp / Q, ib = fp - ia * Q;
-    const int64_t cid = c0 + cell;
-    const int side = (f == 2 || f == 3 || f == 5) ? 1 : 0;
-    const int dir = (f == 2 || f == 4) ? 0 : ((f == 1 || f == 3) ? 1 : 2);
-    const double *lw = lt.lend[side];
-    State own;
-    double fn[5];
-    {
-      double vals[Q], flux[Q];
+  // ---- face task bookkeeping (flux points of the workgroup's cells)
+  int t_cell[T], t_f[T], t_a[T], t_b[T];
+  bool t_on[T];
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+    const int w = tid + 256 * t;
+    t_on[t] = w < nc * NT;
+    const int cell = w / NT, r = w - (w / NT) * NT;
+    t_cell[t] = cell;
+    t_f[t] = r / NFP;
+    const int fp = r - (r / NFP) * NFP;
+    t_a[t] = fp / Q;
+    t_b[t] = fp - (fp / Q) * Q;
+  }
+  double fn[T][5];
+
+  // ---- three passes: one column of F~ at a time through LDS
+  double div[5] = { 0, 0, 0, 0, 0 };
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    __syncthreads();
+    if (node_active) {
+      double col[5];
+      LocalFluxColumn<kDiag>(cu, pu, js, d, col);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) s_x[ncell * S::kScratch + k * PAD + node] = col[k];
+    }
+    __syncthreads();
+    if (node_active) {
+      const int id = d == 0 ? ni : (d == 1 ? nj : nk);
+      const int stride = d == 0 ? Q * Q : (d == 1 ? Q : 1);
+      const int line0 = node - id * stride;
+      const double *sx = s_x + ncell * S::kScratch;
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
+        double acc = div[k];
 #pragma unroll
-        for (int m = 0; m < Q; ++m) {
-          int node = cell * NP + LineNode(Q, f, ia, ib, m);
-          vals[m] = s_u[node * 5 + k];
-          flux[m] = s_f[node * 15 + k * 3 + dir];
+        for (int m = 0; m < Q; ++m) acc = __fma_rn(lt.d[id][m], sx[k * PAD + line0 + m * stride], acc);
+        div[k] = acc;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      const LineIndex li = FaceLineIndex(Q, t_f[t]);
+      if (t_on[t] && li.dir == d) {
+        const int l0 = t_a[t] * li.a_mul + t_b[t] * li.b_mul;
+        const double *lw = lt.lend[li.side];
+        const double *sx = s_x + t_cell[t] * S::kScratch;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+          double vals[Q];
+#pragma unroll
+          for (int m = 0; m < Q; ++m) vals[m] = sx[k * PAD + l0 + m * li.stride];
+          fn[t][k] = Extrapolate<Q>(lw, vals);
         }
+      }
+    }
+  }
+
+  // ---- flux points: traces, upwind flux, flux gap
+  double gap[T][5];
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+    if (!t_on[t]) continue;
+    const int f = t_f[t], ia = t_a[t], ib = t_b[t];
+    const int64_t cid = c0 + t_cell[t];
+    const LineIndex li = FaceLineIndex(Q, f);
+    const double *lw = lt.lend[li.side];
+    State own;
+    {
+      const int l0 = ia * li.a_mul + ib * li.b_mul;
+      const double *su = s_u + t_cell[t] * 5 * PAD;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        double vals[Q];
+#pragma unroll
+        for (int m = 0; m < Q; ++m) vals[m] = su[k * PAD + l0 + m * li.stride];
         own.v[k] = Extrapolate<Q>(lw, vals);
-        fn[k] = Extrapolate<Q>(lw, flux);
       }
     }
     const int32_t nb = a.nbr[cid * 6 + f];
@@ -138,21 +241,26 @@ __global__ __launch_bounds__(256) void FrHexStageKernel(FrHexArgs a, LineTables 
     if (nb >= 0) {
       const int nf = info & 7, orient = (info >> 3) & 7;
       const bool holder = (info >> 6) & 1;
-      int s = (orient & 1) ? ib : ia, t = (orient & 1) ? ia : ib;
+      const int s = (orient & 1) ? ib : ia, tt = (orient & 1) ? ia : ib;
       const int na = (orient & 2) ? Q - 1 - s : s;
-      const int nbb = (orient & 4) ? Q - 1 - t : t;
-      const int nside = (nf == 2 || nf == 3 || nf == 5) ? 1 : 0;
-      const double *nw = lt.lend[nside];
-      const double *src = a.us + (int64_t)nb * NP * 5;
+      const int nbb = (orient & 4) ? Q - 1 - tt : tt;
+      const LineIndex ln = FaceLineIndex(Q, nf);
+      const double *nw = lt.lend[ln.side];
+      const double *src = a.us + ((int64_t)nb * NP + na * ln.a_mul + nbb * ln.b_mul) * 5;
+      double raw[Q][5];
+#pragma unroll
+      for (int m = 0; m < Q; ++m)
+#pragma unroll
+        for (int k = 0; k < 5; ++k) raw[m][k] = src[m * ln.stride * 5 + k];
       State other;
-      double vals[Q];
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
+        double vals[Q];
 #pragma unroll
-        for (int m = 0; m < Q; ++m) vals[m] = src[LineNode(Q, nf, na, nbb, m) * 5 + k];
+        for (int m = 0; m < Q; ++m) vals[m] = raw[m][k];
         other.v[k] = Extrapolate<Q>(nw, vals);
       }
-      fu = UpwindFlux<kRiemann>(frame, holder ? own : other, holder ? other : own);
+      fu = FaceFlux<kRiemann>(frame, holder ? own : other, holder ? other : own);
       sgn = holder ? 1.0 : -1.0;
     } else {
       const int tag = -nb - 1;
@@ -167,68 +275,55 @@ __global__ __launch_bounds__(256) void FrHexStageKernel(FrHexArgs a, LineTables 
       } else if (kind == 3) {
         fu = StateFlux(frame, given);
       } else {
-        fu = UpwindFlux<kRiemann>(frame, own, given);
+        fu = FaceFlux<kRiemann>(frame, own, given);
       }
     }
 #pragma unroll
-    for (int k = 0; k < 5; ++k) s_gap[w * 5 + k] = fu.v[k] * (sgn * scale) - fn[k];
+    for (int k = 0; k < 5; ++k) gap[t][k] = fu.v[k] * (sgn * scale) - fn[t][k];
+  }
+  __syncthreads();  // every read of the F~ columns is done
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+    if (!t_on[t]) continue;
+    const int w = tid + 256 * t;
+    const int r = w - t_cell[t] * NT;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) s_x[t_cell[t] * S::kScratch + k * FPAD + r] = gap[t][k];
   }
   __syncthreads();
 
-  // 3. divergence + lifted gaps + 1/det(J)
-  double r[5];
-  const bool active = tid < nc * NP;
-  if (active) {
-    const int cell = tid / NP, node = tid - cell * NP;
-    const int i = node / (Q * Q), j = (node / Q) % Q, k = node % Q;
-    const double *sf = s_f + cell * NP * 15;
-    const double *sg = s_gap + cell * 6 * NFP * 5;
+  // ---- node phase: lifted gaps, 1/det(J), RK stage, store
+  if (node_active) {
+    const double *sg = s_x + ncell * S::kScratch;
+    const double *su = s_u + ncell * 5 * PAD;
+    const double dt = a.dt;
 #pragma unroll
-    for (int v = 0; v < 5; ++v) {
-      double div = 0;
-#pragma unroll
-      for (int m = 0; m < Q; ++m) {
-        div += lt.d[i][m] * sf[((m * Q + j) * Q + k) * 15 + v * 3 + 0];
-        div += lt.d[j][m] * sf[((i * Q + m) * Q + k) * 15 + v * 3 + 1];
-        div += lt.d[k][m] * sf[((i * Q + j) * Q + m) * 15 + v * 3 + 2];
-      }
-      double corr = sg[((0 * NFP) + i * Q + j) * 5 + v] * lt.gp[0][k]
-                  + sg[((5 * NFP) + i * Q + j) * 5 + v] * lt.gp[1][k]
-                  + sg[((1 * NFP) + i * Q + k) * 5 + v] * lt.gp[0][j]
-                  + sg[((3 * NFP) + i * Q + k) * 5 + v] * lt.gp[1][j]
-                  + sg[((4 * NFP) + j * Q + k) * 5 + v] * lt.gp[0][i]
-                  + sg[((2 * NFP) + j * Q + k) * 5 + v] * lt.gp[1][i];
-      r[v] = -div - corr;
-    }
-    const double inv_det = a.geo[(int64_t)a.geo_id[c0 + cell] * kGeoStride + 9];
-#pragma unroll
-    for (int v = 0; v < 5; ++v) r[v] *= inv_det;
-  }
-  __syncthreads();
-  if (active) {
-#pragma unroll
-    for (int v = 0; v < 5; ++v) s_f[tid * 5 + v] = r[v];
-  }
-  __syncthreads();
-
-  // 4. RK stage in the reference order (temporal/rk.hpp:31-49, ode.hpp:121-128)
-  const double dt = a.dt;
-  for (int e = tid; e < nc * NP * 5; e += 256) {
-    const double res = s_f[e], us = s_u[e];
-    double out;
-    if (a.stage == 0) {
-      out = res;
-    } else {
-      const double next = __dadd_rn(us, __dmul_rn(res, dt));  // residual *= dt; u += residual
-      if (a.stage == 1) {
-        out = next;
-      } else if (a.stage == 2) {
-        out = __dadd_rn(__dmul_rn(a.u0[base + e], 3.0), next) / 4.0;
+    for (int k = 0; k < 5; ++k) {
+      const double *g = sg + k * FPAD;
+      double corr = g[0 * NFP + ni * Q + nj] * lt.gp[0][nk]
+                  + g[5 * NFP + ni * Q + nj] * lt.gp[1][nk]
+                  + g[1 * NFP + ni * Q + nk] * lt.gp[0][nj]
+                  + g[3 * NFP + ni * Q + nk] * lt.gp[1][nj]
+                  + g[4 * NFP + nj * Q + nk] * lt.gp[0][ni]
+                  + g[2 * NFP + nj * Q + nk] * lt.gp[1][ni];
+      const double res = (-div[k] - corr) * inv_det;
+      const double us = su[k * PAD + node];
+      double out;
+      if (a.stage == 0) {
+        out = res;
       } else {
-        out = __dadd_rn(__dmul_rn(next, 2.0), a.u0[base + e]) / 3.0;
+        // temporal/rk.hpp:31-49 with ode.hpp:121-128: residual *= dt; u += residual
+        const double next = __dadd_rn(us, __dmul_rn(res, dt));
+        if (a.stage == 1) {
+          out = next;
+        } else if (a.stage == 2) {
+          out = __dadd_rn(__dmul_rn(a.u0[base + tid * 5 + k], 3.0), next) / 4.0;
+        } else {
+          out = __dadd_rn(__dmul_rn(next, 2.0), a.u0[base + tid * 5 + k]) / 3.0;
+        }
       }
+      a.out[base + tid * 5 + k] = out;
     }
-    a.out[base + e] = out;
   }
 }
 
@@ -263,7 +358,11 @@ hipError_t Launch(const FrHexArgs &args, const LineTables &lt, const BcTable &bc
   constexpr int C = Shape<Q>::kCells;
   int64_t blocks = (args.n_cells + C - 1) / C;
   if (blocks > 0x7fffffff) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((FrHexStageKernel<Q, R>), dim3((unsigned)blocks), dim3(256), 0, stream, args, lt, bc);
+  if (args.diag_metric) {
+    hipLaunchKernelGGL((FrHexStageKernel<Q, R, true>), dim3((unsigned)blocks), dim3(256), 0, stream, args, lt, bc);
+  } else {
+    hipLaunchKernelGGL((FrHexStageKernel<Q, R, false>), dim3((unsigned)blocks), dim3(256), 0, stream, args, lt, bc);
+  }
   return hipGetLastError();
 }
 
