@@ -118,6 +118,7 @@ struct mcfd_solver {
     a.stage = stage;
     a.xcd_remap = 1;
     a.diag_metric = diag_metric ? 1 : 0;
+    a.mixed = scheme.precision == MCFD_PRECISION_MIXED ? 1 : 0;
     switch (stage) {
     case 0: a.us = u0.ptr; a.out = res.ptr; break;
     case 1: a.us = u0.ptr; a.out = u1.ptr; break;
@@ -145,8 +146,8 @@ int mcfd_create(const mcfd_mesh_desc *mesh, const mcfd_scheme_desc *scheme, cons
       throw Failure{ MCFD_ENOTSUP, "only FR_GENERAL on HEXA_8 is implemented in this build" };
     if (scheme->degree < 1 || scheme->degree > 5) throw std::invalid_argument("degree must be 1..5");
     if (scheme->riemann < 0 || scheme->riemann > 2) throw std::invalid_argument("unknown Riemann solver");
-    if (scheme->precision != MCFD_PRECISION_FP64)
-      throw Failure{ MCFD_ENOTSUP, "precision form not implemented" };
+    if (scheme->precision != MCFD_PRECISION_FP64 && scheme->precision != MCFD_PRECISION_MIXED)
+      throw std::invalid_argument("unknown precision form");
     if (n_bcs < 0 || n_bcs > mcfd::kMaxBcs) throw std::invalid_argument("too many boundary conditions");
     h->scheme = *scheme;
     h->q = scheme->degree + 1;

@@ -32,7 +32,13 @@ struct Shape {
   static constexpr int kFacePoints = Q * Q;        // flux points per face
   static constexpr int kFaceTasks = 6 * Q * Q;     // flux points per cell
   static constexpr int kCells = 256 / kPoints;     // cells per workgroup
-  static constexpr int kPad = kPoints + 1;         // odd SoA stride (LDS banks)
+  // Skewed LDS position of solution point (i, j, k): i * kSI + j * kSJ + k.
+  // With kSJ = Q + 1 and kSI = Q (Q + 1) the Q x Q points of any line family
+  // read by one wavefront fall in distinct LDS banks (Q = 4: 20 i + 5 j + k).
+  static constexpr int kSJ = Q + 1;
+  static constexpr int kSI = Q * (Q + 1);
+  static constexpr int kSpan = (Q - 1) * (kSI + kSJ + 1) + 1;
+  static constexpr int kPad = kSpan | 1;           // odd row stride per component
   static constexpr int kFacePad = kFaceTasks + 1;
   static constexpr int kScratch = 5 * (kPad > kFacePad ? kPad : kFacePad);
   static constexpr int kTasks = (kCells * kFaceTasks + 255) / 256;  // per thread
@@ -51,6 +57,19 @@ __device__ __forceinline__ LineIndex FaceLineIndex(int q, int f) {
   li.a_mul = dir == 0 ? q : q * q;
   li.b_mul = dir == 2 ? q : 1;
   li.stride = dir == 0 ? q * q : (dir == 1 ? q : 1);
+  return li;
+}
+// The same line in the skewed LDS layout of Shape<Q>.
+template <int Q>
+__device__ __forceinline__ LineIndex SkewedLineIndex(int f) {
+  constexpr int SI = Q * (Q + 1), SJ = Q + 1;
+  const int dir = (2118 >> (2 * f)) & 3;
+  LineIndex li;
+  li.dir = dir;
+  li.side = (0x2C >> f) & 1;
+  li.a_mul = dir == 0 ? SJ : SI;
+  li.b_mul = dir == 2 ? SJ : 1;
+  li.stride = dir == 0 ? SI : (dir == 1 ? SJ : 1);
   return li;
 }
 
@@ -115,13 +134,21 @@ __device__ __forceinline__ void LocalFluxColumn(State const &c, State const &p, 
   }
 }
 
-template <int Q, int kRiemann, bool kDiag>
+// kMixed: the F~ columns cross LDS as FP32 deviations from the cell's first
+// solution point and are contracted with FP32 FMAs (each row of the
+// derivative matrix sums to zero, so the FP64 reference value drops out of
+// the divergence); each direction's FP32 partial is accumulated in FP64.
+// States, pressure recovery, traces, the Riemann solve, the flux gaps and
+// the RK combination stay FP64.
+template <int Q, int kRiemann, bool kDiag, bool kMixed>
 __global__ __launch_bounds__(256) void FrHexStageKernel(FrHexArgs a, LineTables lt, BcTable bc) {
   using S = Shape<Q>;
   constexpr int NP = S::kPoints, NFP = S::kFacePoints, NT = S::kFaceTasks, C = S::kCells;
   constexpr int PAD = S::kPad, FPAD = S::kFacePad, T = S::kTasks;
   __shared__ double s_u[C * 5 * PAD];         // states, SoA per cell
   __shared__ double s_x[C * S::kScratch];     // F~ column per pass, then flux gaps
+  __shared__ double s_ref[C * 5];             // mixed: FP64 column value at node 0
+  float *s_x32 = reinterpret_cast<float *>(s_x);
 
   const int tid = threadIdx.x;
   const int64_t c0 = LogicalBlock(a.xcd_remap) * C;
@@ -133,6 +160,7 @@ __global__ __launch_bounds__(256) void FrHexStageKernel(FrHexArgs a, LineTables 
   const bool node_active = tid < nc * NP;
   const int ncell = tid / NP, node = tid - (tid / NP) * NP;
   const int ni = node / (Q * Q), nj = (node / Q) % Q, nk = node % Q;
+  const int spos = ni * S::kSI + nj * S::kSJ + nk;  // skewed LDS position
   State cu, pu;
   const double *js = nullptr;
   double inv_det = 0;
@@ -143,7 +171,7 @@ __global__ __launch_bounds__(256) void FrHexStageKernel(FrHexArgs a, LineTables 
     js = a.geo + (int64_t)a.geo_id[c0 + ncell] * kGeoStride;
     inv_det = js[9];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) s_u[(ncell * 5 + k) * PAD + node] = cu.v[k];
+    for (int k = 0; k < 5; ++k) s_u[(ncell * 5 + k) * PAD + spos] = cu.v[k];
   }
 
   // ---- face task bookkeeping (flux points of the workgroup's cells)
@@ -167,39 +195,76 @@ __global__ __launch_bounds__(256) void FrHexStageKernel(FrHexArgs a, LineTables 
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     __syncthreads();
+    double col[5];
     if (node_active) {
-      double col[5];
       LocalFluxColumn<kDiag>(cu, pu, js, d, col);
+      if constexpr (kMixed) {
+        if (node == 0) {
 #pragma unroll
-      for (int k = 0; k < 5; ++k) s_x[ncell * S::kScratch + k * PAD + node] = col[k];
+          for (int k = 0; k < 5; ++k) s_ref[ncell * 5 + k] = col[k];
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) s_x[ncell * S::kScratch + k * PAD + spos] = col[k];
+      }
+    }
+    if constexpr (kMixed) {
+      __syncthreads();
+      if (node_active) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+          s_x32[ncell * S::kScratch + k * PAD + spos] = (float)(col[k] - s_ref[ncell * 5 + k]);
+      }
     }
     __syncthreads();
     if (node_active) {
       const int id = d == 0 ? ni : (d == 1 ? nj : nk);
-      const int stride = d == 0 ? Q * Q : (d == 1 ? Q : 1);
-      const int line0 = node - id * stride;
-      const double *sx = s_x + ncell * S::kScratch;
+      const int stride = d == 0 ? S::kSI : (d == 1 ? S::kSJ : 1);
+      const int line0 = spos - id * stride;
+      if constexpr (kMixed) {
+        const float *sx = s_x32 + ncell * S::kScratch;
 #pragma unroll
-      for (int k = 0; k < 5; ++k) {
-        double acc = div[k];
+        for (int k = 0; k < 5; ++k) {
+          float acc = 0.0f;
 #pragma unroll
-        for (int m = 0; m < Q; ++m) acc = __fma_rn(lt.d[id][m], sx[k * PAD + line0 + m * stride], acc);
-        div[k] = acc;
+          for (int m = 0; m < Q; ++m) acc = __fmaf_rn((float)lt.d[id][m], sx[k * PAD + line0 + m * stride], acc);
+          div[k] += (double)acc;
+        }
+      } else {
+        const double *sx = s_x + ncell * S::kScratch;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+          double acc = div[k];
+#pragma unroll
+          for (int m = 0; m < Q; ++m) acc = __fma_rn(lt.d[id][m], sx[k * PAD + line0 + m * stride], acc);
+          div[k] = acc;
+        }
       }
     }
 #pragma unroll
     for (int t = 0; t < T; ++t) {
-      const LineIndex li = FaceLineIndex(Q, t_f[t]);
+      const LineIndex li = SkewedLineIndex<Q>(t_f[t]);
       if (t_on[t] && li.dir == d) {
         const int l0 = t_a[t] * li.a_mul + t_b[t] * li.b_mul;
         const double *lw = lt.lend[li.side];
-        const double *sx = s_x + t_cell[t] * S::kScratch;
+        if constexpr (kMixed) {
+          const float *sx = s_x32 + t_cell[t] * S::kScratch;
 #pragma unroll
-        for (int k = 0; k < 5; ++k) {
-          double vals[Q];
+          for (int k = 0; k < 5; ++k) {
+            float acc = (float)lw[0] * sx[k * PAD + l0];
 #pragma unroll
-          for (int m = 0; m < Q; ++m) vals[m] = sx[k * PAD + l0 + m * li.stride];
-          fn[t][k] = Extrapolate<Q>(lw, vals);
+            for (int m = 1; m < Q; ++m) acc = __fmaf_rn((float)lw[m], sx[k * PAD + l0 + m * li.stride], acc);
+            fn[t][k] = s_ref[t_cell[t] * 5 + k] + (double)acc;
+          }
+        } else {
+          const double *sx = s_x + t_cell[t] * S::kScratch;
+#pragma unroll
+          for (int k = 0; k < 5; ++k) {
+            double vals[Q];
+#pragma unroll
+            for (int m = 0; m < Q; ++m) vals[m] = sx[k * PAD + l0 + m * li.stride];
+            fn[t][k] = Extrapolate<Q>(lw, vals);
+          }
         }
       }
     }
@@ -212,7 +277,7 @@ __global__ __launch_bounds__(256) void FrHexStageKernel(FrHexArgs a, LineTables 
     if (!t_on[t]) continue;
     const int f = t_f[t], ia = t_a[t], ib = t_b[t];
     const int64_t cid = c0 + t_cell[t];
-    const LineIndex li = FaceLineIndex(Q, f);
+    const LineIndex li = SkewedLineIndex<Q>(f);
     const double *lw = lt.lend[li.side];
     State own;
     {
@@ -307,7 +372,7 @@ __global__ __launch_bounds__(256) void FrHexStageKernel(FrHexArgs a, LineTables 
                   + g[4 * NFP + nj * Q + nk] * lt.gp[0][ni]
                   + g[2 * NFP + nj * Q + nk] * lt.gp[1][ni];
       const double res = (-div[k] - corr) * inv_det;
-      const double us = su[k * PAD + node];
+      const double us = su[k * PAD + spos];
       double out;
       if (a.stage == 0) {
         out = res;
@@ -358,10 +423,19 @@ hipError_t Launch(const FrHexArgs &args, const LineTables &lt, const BcTable &bc
   constexpr int C = Shape<Q>::kCells;
   int64_t blocks = (args.n_cells + C - 1) / C;
   if (blocks > 0x7fffffff) return hipErrorInvalidValue;
-  if (args.diag_metric) {
-    hipLaunchKernelGGL((FrHexStageKernel<Q, R, true>), dim3((unsigned)blocks), dim3(256), 0, stream, args, lt, bc);
+  const dim3 grid((unsigned)blocks), block(256);
+  if (args.mixed) {
+    if (args.diag_metric) {
+      hipLaunchKernelGGL((FrHexStageKernel<Q, R, true, true>), grid, block, 0, stream, args, lt, bc);
+    } else {
+      hipLaunchKernelGGL((FrHexStageKernel<Q, R, false, true>), grid, block, 0, stream, args, lt, bc);
+    }
   } else {
-    hipLaunchKernelGGL((FrHexStageKernel<Q, R, false>), dim3((unsigned)blocks), dim3(256), 0, stream, args, lt, bc);
+    if (args.diag_metric) {
+      hipLaunchKernelGGL((FrHexStageKernel<Q, R, true, false>), grid, block, 0, stream, args, lt, bc);
+    } else {
+      hipLaunchKernelGGL((FrHexStageKernel<Q, R, false, false>), grid, block, 0, stream, args, lt, bc);
+    }
   }
   return hipGetLastError();
 }

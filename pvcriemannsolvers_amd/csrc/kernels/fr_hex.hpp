@@ -48,6 +48,7 @@ struct FrHexArgs {
   int32_t stage;         // 0 residual, 1..3 SSP-RK3 stage
   int32_t xcd_remap;     // remap block ids so each XCD sweeps a contiguous range
   int32_t diag_metric;   // every J* is diagonal (axis-aligned cells)
+  int32_t mixed;         // FP32 flux-column contractions (mixed-precision form)
 };
 
 // Launches one residual / RK-stage pass for Q points per direction and the

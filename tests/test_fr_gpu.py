@@ -148,3 +148,29 @@ def test_free_stream_and_conservation_64cubed():
     assert np.abs(total).max() < 1e-9 * np.abs(r).max() * r.shape[0]
     # deterministic: a second evaluation is bitwise identical
     assert np.array_equal(r.reshape(-1), g.get_residual_column())
+
+
+@pytest.mark.parametrize("degree", [3, 4])
+def test_mixed_precision_form(degree):
+    """Mixed form: FP32 flux-column contractions; FP64 states, traces, Roe, RK.
+
+    Tolerances (stated, not bit-level): residual within 1e-5 of the FP64
+    oracle relative to its max norm; after 10 RK3 steps at CFL 0.1 the
+    density and energy differ from the oracle by <= 1e-8 in the L2 norm of
+    mesh/part.hpp:1022-1036 (sqrt of sum of w_q * du^2).
+    """
+    mesh = meshgen.box_hex(5, 5, 5, periodic=(True, True, True))
+    o = ol.FrOracle(mesh, q=degree + 1, riemann=2)
+    g = spatial.FiniteElement(mesh, degree=degree, riemann="roe", precision="mixed")
+    u = density_wave(o.node_coords(), seed=6)
+    g.set_solution_column(u.reshape(-1))
+    assert rel(g.get_residual_column(), o.residual(u).reshape(-1)) < 1e-5
+    dt = o.time_step(u.reshape(-1), 0.1)
+    ref = o.rk3(u.reshape(-1), dt, n_steps=10).reshape(u.shape)
+    for s in range(10):
+        g.rk3_update(s * dt, dt)
+    got = g.get_solution_column().reshape(u.shape)
+    w = o.node_weights()
+    for k in (0, 4):
+        l2 = np.sqrt((w * (got[..., k] - ref[..., k]) ** 2).sum())
+        assert l2 <= 1e-8, (k, l2)
