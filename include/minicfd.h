@@ -168,6 +168,42 @@ int mcfd_synchronize(mcfd_solver *h);
 int mcfd_time_rk3(mcfd_solver *h, double dt, int32_t n_steps,
                   double *total_ms, double *stage_kernel_ms);
 
+
+/* ---- batched form
+ * Many independent cases (own mesh, own degree 1..3, own BC table) advanced
+ * together: cases are grouped by degree, each SSP-RK3 stage is one
+ * persistent launch over the blocks of every group, and the three stages of
+ * a step are captured once in a hipGraph and replayed.  Each case keeps its
+ * own time step (the reference advances one System per case through
+ * RungeKutta<3>::Update, temporal/rk.hpp:31-49).  No reference counterpart
+ * exists for the batching itself. */
+typedef struct mcfd_batch mcfd_batch;
+/* bcs[bc_offsets[c] .. bc_offsets[c+1]) is case c's BC table, indexed by its
+ * mesh's boundary tags; scheme->degree is ignored in favour of degrees[c]. */
+int mcfd_batch_create(int32_t n_cases, const mcfd_mesh_desc *meshes, const int32_t *degrees,
+                      const mcfd_scheme_desc *scheme, const mcfd_bc_desc *bcs,
+                      const int32_t *bc_offsets, mcfd_batch **out);
+int mcfd_batch_destroy(mcfd_batch *b);
+int mcfd_batch_case_sizes(const mcfd_batch *b, int32_t case_id, int64_t *n_cells, int32_t *n_per_cell);
+/* Solution points of one case, [n_cells][N][3]. */
+int mcfd_batch_sample_points(const mcfd_batch *b, int32_t case_id, double *xyz);
+/* System::SetSolutionColumn / GetSolutionColumn of one case. */
+int mcfd_batch_set_solution_column(mcfd_batch *b, int32_t case_id, const double *column);
+int mcfd_batch_get_solution_column(mcfd_batch *b, int32_t case_id, double *column);
+/* FiniteElement::GetTimeStep (fem.hpp:417-435) of every case at CFL `cfl`. */
+int mcfd_batch_get_time_steps(mcfd_batch *b, double cfl, double *dt_per_case);
+/* Time step of every case used by the following steps. */
+int mcfd_batch_set_time_steps(mcfd_batch *b, const double *dt_per_case);
+/* Times every tile variant (cells per workgroup) of every degree group with
+ * `repeats` stage launches, keeps the fastest and re-captures the step
+ * graph; tiles[g] = 16 * Q + variant for each degree group in ascending
+ * degree (may be null). */
+int mcfd_batch_autotune(mcfd_batch *b, int32_t repeats, int32_t *tiles);
+/* n_steps replays of the captured SSP-RK3 step (stream-ordered). */
+int mcfd_batch_rk3_steps(mcfd_batch *b, int32_t n_steps);
+int mcfd_batch_synchronize(mcfd_batch *b);
+int mcfd_batch_time_rk3(mcfd_batch *b, int32_t n_steps, double *total_ms);
+
 #ifdef __cplusplus
 }
 #endif

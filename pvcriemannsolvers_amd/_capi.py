@@ -63,6 +63,19 @@ SIGNATURES = {
     "mcfd_faces": [ctypes.c_void_p, _i64p, _i64p, _i32p, _i32p, _i32p],
     "mcfd_synchronize": [ctypes.c_void_p],
     "mcfd_time_rk3": [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32, _dp, _dp],
+    "mcfd_batch_create": [ctypes.c_int32, ctypes.POINTER(MeshDesc), _i32p, ctypes.POINTER(SchemeDesc),
+                          ctypes.POINTER(BcDesc), _i32p, ctypes.POINTER(ctypes.c_void_p)],
+    "mcfd_batch_destroy": [ctypes.c_void_p],
+    "mcfd_batch_case_sizes": [ctypes.c_void_p, ctypes.c_int32, _i64p, _i32p],
+    "mcfd_batch_sample_points": [ctypes.c_void_p, ctypes.c_int32, _dp],
+    "mcfd_batch_set_solution_column": [ctypes.c_void_p, ctypes.c_int32, _dp],
+    "mcfd_batch_get_solution_column": [ctypes.c_void_p, ctypes.c_int32, _dp],
+    "mcfd_batch_get_time_steps": [ctypes.c_void_p, ctypes.c_double, _dp],
+    "mcfd_batch_set_time_steps": [ctypes.c_void_p, _dp],
+    "mcfd_batch_autotune": [ctypes.c_void_p, ctypes.c_int32, _i32p],
+    "mcfd_batch_rk3_steps": [ctypes.c_void_p, ctypes.c_int32],
+    "mcfd_batch_synchronize": [ctypes.c_void_p],
+    "mcfd_batch_time_rk3": [ctypes.c_void_p, ctypes.c_int32, _dp],
 }
 
 _lib = None

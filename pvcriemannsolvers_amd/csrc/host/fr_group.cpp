@@ -1,0 +1,218 @@
+// FrHexGroup: host construction and device upload of one FR mesh.
+#include "fr_group.hpp"
+
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+
+#include "tables.hpp"
+
+namespace mcfd {
+
+namespace {
+
+// Inverse of an orientation code (see ApplyOrient in hex_connectivity.hpp).
+int InverseOrient(int code) {
+  const int q = 4;
+  for (int c = 0; c < 8; ++c) {
+    bool ok = true;
+    for (int a = 0; a < q && ok; ++a)
+      for (int b = 0; b < q && ok; ++b) {
+        int a1, b1, a2, b2;
+        ApplyOrient(code, q, a, b, &a1, &b1);
+        ApplyOrient(c, q, a1, b1, &a2, &b2);
+        ok = (a2 == a && b2 == b);
+      }
+    if (ok) return c;
+  }
+  return -1;
+}
+
+}  // namespace
+
+void FrHexGroup::Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &scheme,
+                       const std::vector<int32_t> &bc_kinds, const std::vector<double> &bc_states) {
+  if (scheme.degree < 1 || scheme.degree > 5) throw std::invalid_argument("degree must be 1..5");
+  if (scheme.riemann < 0 || scheme.riemann > 2) throw std::invalid_argument("unknown Riemann solver");
+  if (scheme.precision != MCFD_PRECISION_FP64 && scheme.precision != MCFD_PRECISION_MIXED)
+    throw std::invalid_argument("unknown precision form");
+  degree = scheme.degree;
+  riemann = scheme.riemann;
+  mixed = scheme.precision == MCFD_PRECISION_MIXED;
+  q = degree + 1;
+  np = q * q * q;
+  n_cells = mesh.n_cells;
+  const int n_bcs = (int)bc_kinds.size();
+  for (int64_t b = 0; b < mesh.n_bfaces; ++b)
+    if (mesh.bface_tags[b] >= n_bcs) throw std::invalid_argument("boundary tag without a BC entry");
+  for (auto k : bc_kinds)
+    if (k < 0 || k > 3) throw std::invalid_argument("unknown BC kind");
+  h_bc_kind = bc_kinds;
+  h_bc_state = bc_states;
+  if (h_bc_kind.empty()) {  // keep the device arrays non-empty
+    h_bc_kind.push_back(1);
+    h_bc_state.assign(5, 0.0);
+  }
+  conn = BuildHexConnectivity(mesh);
+
+  // line tables
+  x1d.resize(q);
+  std::vector<double> w(q);
+  GaussRule(scheme.nodes == MCFD_NODES_LOBATTO, q, x1d.data(), w.data());
+  std::memset(&lt, 0, sizeof(lt));
+  for (int i = 0; i < q; ++i)
+    for (int m = 0; m < q; ++m) lt.d[i][m] = LagrangeDerivative(q, x1d.data(), m, x1d[i]);
+  for (int m = 0; m < q; ++m) {
+    lt.lend[0][m] = LagrangeValue(q, x1d.data(), m, -1.0);
+    lt.lend[1][m] = LagrangeValue(q, x1d.data(), m, +1.0);
+    lt.gp[0][m] = CorrectionLeftDerivative(degree, scheme.c_next, x1d[m]);
+    lt.gp[1][m] = CorrectionRightDerivative(degree, scheme.c_next, x1d[m]);
+  }
+
+  // geometry records, deduplicated bitwise
+  const int64_t nc = mesh.n_cells;
+  std::map<std::array<double, kGeoStride>, int32_t> geo_ids;
+  h_geo.clear();
+  h_geo_id.assign(nc, 0);
+  corner0.resize(nc);
+  for (int64_t c = 0; c < nc; ++c) {
+    const auto &j = conn.jac[c];
+    double d = conn.det[c];
+    // J* = det(J) J^{-1} is the adjugate of J
+    std::array<double, kGeoStride> g{};
+    g[0] = j[4] * j[8] - j[5] * j[7]; g[1] = j[2] * j[7] - j[1] * j[8]; g[2] = j[1] * j[5] - j[2] * j[4];
+    g[3] = j[5] * j[6] - j[3] * j[8]; g[4] = j[0] * j[8] - j[2] * j[6]; g[5] = j[2] * j[3] - j[0] * j[5];
+    g[6] = j[3] * j[7] - j[4] * j[6]; g[7] = j[1] * j[6] - j[0] * j[7]; g[8] = j[0] * j[4] - j[1] * j[3];
+    g[9] = 1.0 / d;
+    for (int f = 0; f < 6; ++f) {
+      int dir = kHexFaceNormalDir[f];
+      double n = std::sqrt(g[dir] * g[dir] + g[3 + dir] * g[3 + dir] + g[6 + dir] * g[6 + dir]);
+      g[10 + f] = kHexFaceSide[f] ? n : -n;
+    }
+    g[16] = std::cbrt(8 * d) * 0.5;  // mesh/part.hpp:230, volume = 8 det(J)
+    auto it = geo_ids.find(g);
+    if (it == geo_ids.end()) {
+      it = geo_ids.emplace(g, (int32_t)geo_ids.size()).first;
+      h_geo.insert(h_geo.end(), g.begin(), g.end());
+    }
+    h_geo_id[c] = it->second;
+    int64_t n0 = mesh.cell_nodes[8 * c];
+    corner0[c] = { mesh.coords[3 * n0], mesh.coords[3 * n0 + 1], mesh.coords[3 * n0 + 2] };
+  }
+  diag_metric = true;
+  for (size_t r = 0; r < h_geo.size(); r += kGeoStride)
+    for (int e = 0; e < 9; ++e)
+      if (e % 4 != 0 && h_geo[r + e] != 0.0) diag_metric = false;
+
+  // per-cell face records
+  std::map<std::array<double, kFrameStride>, int32_t> frame_ids;
+  h_frames.clear();
+  h_nbr.assign(6 * nc, INT32_MIN);
+  h_finfo.assign(6 * nc, 0);
+  for (const auto &face : conn.faces) {
+    std::array<double, kFrameStride> fr;
+    std::copy(face.frame, face.frame + 6, fr.begin());
+    auto it = frame_ids.find(fr);
+    if (it == frame_ids.end()) {
+      it = frame_ids.emplace(fr, (int32_t)frame_ids.size()).first;
+      h_frames.insert(h_frames.end(), fr.begin(), fr.end());
+    }
+    int32_t fid = it->second;
+    if (fid >= (1 << 24)) throw Failure{ MCFD_ENOTSUP, "too many distinct face frames" };
+    int64_t hs = face.holder * 6 + face.holder_face;
+    if (face.sharer >= 0) {
+      int64_t ss = face.sharer * 6 + face.sharer_face;
+      h_nbr[hs] = (int32_t)face.sharer;
+      h_finfo[hs] = (fid << 7) | (1 << 6) | (face.orient << 3) | face.sharer_face;
+      h_nbr[ss] = (int32_t)face.holder;
+      h_finfo[ss] = (fid << 7) | (InverseOrient(face.orient) << 3) | face.holder_face;
+    } else {
+      h_nbr[hs] = -(1 + face.tag);
+      h_finfo[hs] = (fid << 7) | (1 << 6);
+    }
+  }
+  for (auto v : h_nbr)
+    if (v == INT32_MIN) throw std::invalid_argument("cell face without a face record");
+}
+
+void FrHexGroup::Upload(hipStream_t stream) {
+  size_t cs = ColumnSize();
+  u0.Allocate(cs); u1.Allocate(cs); u2.Allocate(cs); res.Allocate(cs);
+  Check(hipMemsetAsync(u0.ptr, 0, cs * sizeof(double), stream), "memset");
+  nbr.Upload(h_nbr, stream);
+  finfo.Upload(h_finfo, stream);
+  geo_id.Upload(h_geo_id, stream);
+  geo.Upload(h_geo, stream);
+  frames.Upload(h_frames, stream);
+  bc_kind.Upload(h_bc_kind, stream);
+  bc_state.Upload(h_bc_state, stream);
+  if (!h_cell_case.empty()) cell_case.Upload(h_cell_case, stream);
+}
+
+FrHexArgs FrHexGroup::Args(int stage, double dt) const {
+  FrHexArgs a;
+  a.u0 = u0.ptr;
+  a.nbr = nbr.ptr; a.finfo = finfo.ptr; a.geo_id = geo_id.ptr;
+  a.geo = geo.ptr; a.frames = frames.ptr;
+  a.bc_kind = bc_kind.ptr; a.bc_state = bc_state.ptr;
+  a.cell_case = cell_case.ptr;
+  a.case_dt = cell_case.ptr ? case_dt.ptr : nullptr;
+  a.n_cells = n_cells;
+  a.dt = dt;
+  a.stage = stage;
+  a.xcd_remap = 1;
+  a.diag_metric = diag_metric ? 1 : 0;
+  a.mixed = mixed ? 1 : 0;
+  switch (stage) {
+  case 0: a.us = u0.ptr; a.out = res.ptr; break;
+  case 1: a.us = u0.ptr; a.out = u1.ptr; break;
+  case 2: a.us = u1.ptr; a.out = u2.ptr; break;
+  default: a.us = u2.ptr; a.out = u0.ptr; break;
+  }
+  return a;
+}
+
+void FrHexGroup::SamplePoints(double *xyz) const {
+  for (int64_t c = 0; c < n_cells; ++c) {
+    const auto &j = conn.jac[c];
+    const auto &x0 = corner0[c];
+    for (int i = 0; i < q; ++i)
+      for (int jj = 0; jj < q; ++jj)
+        for (int k = 0; k < q; ++k) {
+          double s[3] = { x1d[i] + 1, x1d[jj] + 1, x1d[k] + 1 };
+          double *p = xyz + ((c * np) + (i * q + jj) * q + k) * 3;
+          for (int d = 0; d < 3; ++d) p[d] = x0[d] + s[0] * j[d] + s[1] * j[3 + d] + s[2] * j[6 + d];
+        }
+  }
+}
+
+void FrHexGroup::Faces(int64_t *holder, int64_t *sharer, int32_t *holder_face, int32_t *sharer_face,
+                       int32_t *fp_ijk) const {
+  const int nfp = q * q;
+  for (size_t f = 0; f < conn.faces.size(); ++f) {
+    const auto &face = conn.faces[f];
+    if (holder) holder[f] = face.holder;
+    if (sharer) sharer[f] = face.sharer;
+    if (holder_face) holder_face[f] = face.holder_face;
+    if (sharer_face) sharer_face[f] = face.sharer_face;
+    if (!fp_ijk) continue;
+    for (int i = 0; i < q; ++i)
+      for (int j = 0; j < q; ++j) {
+        int a, b;
+        ApplyOrient(face.quad_to_holder, q, i, j, &a, &b);
+        int fh = face.holder_face;
+        int32_t *o = fp_ijk + (f * nfp + i * q + j) * 2;
+        o[0] = HexLineNode(q, fh, a, b, kHexFaceSide[fh] ? q - 1 : 0);
+        o[1] = -1;
+        if (face.sharer >= 0) {
+          int a2, b2, fs = face.sharer_face;
+          ApplyOrient(face.orient, q, a, b, &a2, &b2);
+          o[1] = HexLineNode(q, fs, a2, b2, kHexFaceSide[fs] ? q - 1 : 0);
+        }
+      }
+  }
+}
+
+}  // namespace mcfd

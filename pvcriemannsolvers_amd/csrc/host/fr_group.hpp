@@ -1,0 +1,48 @@
+// One mesh discretised by nodal FR on affine hexahedra: host connectivity,
+// line tables, geometry/frame records and the device arrays the stage kernel
+// reads.  A solver handle owns one group; a batch owns one per degree.
+#ifndef MINICFD_HOST_FR_GROUP_HPP_
+#define MINICFD_HOST_FR_GROUP_HPP_
+
+#include <array>
+#include <cstdint>
+#include <vector>
+
+#include "../kernels/fr_hex.hpp"
+#include "common.hpp"
+#include "hex_connectivity.hpp"
+
+namespace mcfd {
+
+struct FrHexGroup {
+  int q = 0, np = 0, degree = 0, riemann = 0;
+  bool mixed = false, diag_metric = false;
+  int64_t n_cells = 0;
+  HexConnectivity conn;
+  LineTables lt;
+  std::vector<double> x1d;
+  std::vector<std::array<double, 3>> corner0;  // node 0 of each cell
+  // host copies uploaded by Upload()
+  std::vector<int32_t> h_nbr, h_finfo, h_geo_id, h_bc_kind, h_cell_case;
+  std::vector<double> h_geo, h_frames, h_bc_state;
+  DeviceArray<double> u0, u1, u2, res;
+  DeviceArray<int32_t> nbr, finfo, geo_id, bc_kind, cell_case;
+  DeviceArray<double> geo, frames, bc_state, case_dt;
+
+  // Host-side construction; bc_kind/bc_state index the mesh's boundary tags.
+  void Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &scheme,
+             const std::vector<int32_t> &bc_kinds, const std::vector<double> &bc_states);
+  // Device allocation and upload on `stream` (not synchronised).
+  void Upload(hipStream_t stream);
+
+  size_t ColumnSize() const { return size_t(n_cells) * np * 5; }
+  // Kernel arguments of one pass: stage 0 residual, 1..3 SSP-RK3 stages.
+  FrHexArgs Args(int stage, double dt) const;
+  void SamplePoints(double *xyz) const;
+  void Faces(int64_t *holder, int64_t *sharer, int32_t *holder_face, int32_t *sharer_face,
+             int32_t *fp_ijk) const;
+};
+
+}  // namespace mcfd
+
+#endif  // MINICFD_HOST_FR_GROUP_HPP_

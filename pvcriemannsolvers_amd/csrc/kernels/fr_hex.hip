@@ -1,23 +1,26 @@
-// Nodal flux reconstruction on affine hexahedra for CDNA4 (gfx950), FP64.
+// Nodal flux reconstruction on affine hexahedra for CDNA4 (gfx950).
 //
 // One pass = residual of spatial/fr/general.hpp (volume term :271-288, face
 // correction :296-315,409-434, boundary corrections :435-640, det(J) scaling
 // :249-268) fused with one SSP-RK3 stage of temporal/rk.hpp:31-49.
 //
-// Work decomposition: a 256-thread workgroup owns C consecutive cells
-// (C * Q^3 <= 256 solution points).  Each pass
-//   0. streams the cells' states (contiguous in the reference column layout)
-//      into LDS with coalesced flat loads;
-//   1. computes the local flux F~ = F J* per solution point into LDS;
-//   2. computes, per face point of its own 6 faces, the own trace (LDS), the
-//      neighbour trace (global loads through L2 -- the neighbour's line of Q
-//      points), the upwind flux and the flux gap; each interior face is thus
-//      solved by both of its cells with bit-identical inputs, so the scheme
-//      stays conservative without atomics or a separate face pass;
-//   3. sums the sum-factorised divergence (three Q-point contractions) and
-//      the six lifted gaps per solution point and scales by 1/det(J);
-//   4. combines with the RK stage in the reference operation order and
-//      streams the result out with coalesced flat stores.
+// Work decomposition: a 256-thread workgroup owns C consecutive cells.  Each
+// pass
+//   0. loads the cells' states (one thread per solution point) and keeps a
+//      skewed structure-of-arrays copy in LDS;
+//   1. three passes, one per reference direction: the F~ = F J* column goes
+//      through LDS, every solution point accumulates its sum-factorised
+//      Q-point derivative and every flux point normal to that direction
+//      extrapolates the column to the face;
+//   2. per flux point of the cells' 6 faces: own trace (LDS), neighbour trace
+//      (the neighbour's line of Q points, gathered through L2), upwind flux
+//      and flux gap.  Each interior face is solved by both of its cells with
+//      bit-identical inputs, so the scheme stays conservative without
+//      atomics or a separate face pass;
+//   3. per solution point: lifted gaps, 1/det(J), RK stage in the
+//      reference's operation order, store.
+// The same body serves the single-mesh launch and the persistent batched
+// launch that walks a work list spanning several polynomial degrees.
 #include "fr_hex.hpp"
 
 #include "euler.hpp"
@@ -26,12 +29,13 @@ namespace mcfd {
 
 namespace {
 
-template <int Q>
+template <int Q, int Cells = 256 / (Q * Q * Q)>
 struct Shape {
   static constexpr int kPoints = Q * Q * Q;        // solution points per cell
   static constexpr int kFacePoints = Q * Q;        // flux points per face
   static constexpr int kFaceTasks = 6 * Q * Q;     // flux points per cell
-  static constexpr int kCells = 256 / kPoints;     // cells per workgroup
+  static constexpr int kCells = Cells;             // cells per workgroup
+  static_assert(Cells * Q * Q * Q <= 256, "one thread per solution point");
   // Skewed LDS position of solution point (i, j, k): i * kSI + j * kSJ + k.
   // With kSJ = Q + 1 and kSI = Q (Q + 1) the Q x Q points of any line family
   // read by one wavefront fall in distinct LDS banks (Q = 4: 20 i + 5 j + k).
@@ -43,6 +47,7 @@ struct Shape {
   static constexpr int kScratch = 5 * (kPad > kFacePad ? kPad : kFacePad);
   static constexpr int kTasks = (kCells * kFaceTasks + 255) / 256;  // per thread
   static_assert(kCells >= 1, "at most 256 solution points per cell");
+  static constexpr int kLdsDoubles = kCells * 5 * kPad + kCells * kScratch + kCells * 5;
 };
 
 // Line of solution points through in-face point (a, b) of local face f:
@@ -140,18 +145,19 @@ __device__ __forceinline__ void LocalFluxColumn(State const &c, State const &p, 
 // the divergence); each direction's FP32 partial is accumulated in FP64.
 // States, pressure recovery, traces, the Riemann solve, the flux gaps and
 // the RK combination stay FP64.
-template <int Q, int kRiemann, bool kDiag, bool kMixed>
-__global__ __launch_bounds__(256) void FrHexStageKernel(FrHexArgs a, LineTables lt, BcTable bc) {
-  using S = Shape<Q>;
+template <int Q, int Cells, int kRiemann, bool kDiag, bool kMixed>
+__device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTables &lt, int64_t block,
+                                               double *lds) {
+  using S = Shape<Q, Cells>;
   constexpr int NP = S::kPoints, NFP = S::kFacePoints, NT = S::kFaceTasks, C = S::kCells;
   constexpr int PAD = S::kPad, FPAD = S::kFacePad, T = S::kTasks;
-  __shared__ double s_u[C * 5 * PAD];         // states, SoA per cell
-  __shared__ double s_x[C * S::kScratch];     // F~ column per pass, then flux gaps
-  __shared__ double s_ref[C * 5];             // mixed: FP64 column value at node 0
+  double *s_u = lds;                          // states, SoA per cell
+  double *s_x = lds + C * 5 * PAD;            // F~ column per pass, then flux gaps
+  double *s_ref = s_x + C * S::kScratch;      // mixed: FP64 column value at node 0
   float *s_x32 = reinterpret_cast<float *>(s_x);
 
   const int tid = threadIdx.x;
-  const int64_t c0 = LogicalBlock(a.xcd_remap) * C;
+  const int64_t c0 = block * C;
   if (c0 >= a.n_cells) return;
   const int nc = (int)min<int64_t>(C, a.n_cells - c0);
   const int64_t base = c0 * NP * 5;
@@ -329,10 +335,10 @@ __global__ __launch_bounds__(256) void FrHexStageKernel(FrHexArgs a, LineTables 
       sgn = holder ? 1.0 : -1.0;
     } else {
       const int tag = -nb - 1;
-      const int kind = bc.kind[tag];
+      const int kind = a.bc_kind[tag];
       State given;
 #pragma unroll
-      for (int k = 0; k < 5; ++k) given.v[k] = bc.state[tag][k];
+      for (int k = 0; k < 5; ++k) given.v[k] = a.bc_state[tag * 5 + k];
       if (kind == 1) {
         fu = WallFlux(frame, own);
       } else if (kind == 2) {
@@ -361,7 +367,7 @@ __global__ __launch_bounds__(256) void FrHexStageKernel(FrHexArgs a, LineTables 
   if (node_active) {
     const double *sg = s_x + ncell * S::kScratch;
     const double *su = s_u + ncell * 5 * PAD;
-    const double dt = a.dt;
+    const double dt = a.case_dt ? a.case_dt[a.cell_case[c0 + ncell]] : a.dt;
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
       const double *g = sg + k * FPAD;
@@ -392,6 +398,58 @@ __global__ __launch_bounds__(256) void FrHexStageKernel(FrHexArgs a, LineTables 
   }
 }
 
+template <int Q, int Cells, int kRiemann, bool kDiag, bool kMixed>
+__global__ __launch_bounds__(256) void FrHexStageKernel(FrHexArgs a, LineTables lt) {
+  __shared__ double lds[Shape<Q, Cells>::kLdsDoubles];
+  FrHexStageBody<Q, Cells, kRiemann, kDiag, kMixed>(a, lt, LogicalBlock(a.xcd_remap), lds);
+}
+
+// Tile variants per Q (cells per workgroup) that the batched launch and its
+// autotuner choose from; variant 0 is the single-mesh default.
+template <int Q, int V> struct TileOf;
+template <> struct TileOf<2, 0> { static constexpr int kCells = 32; };
+template <> struct TileOf<2, 1> { static constexpr int kCells = 16; };
+template <> struct TileOf<2, 2> { static constexpr int kCells = 8; };
+template <> struct TileOf<3, 0> { static constexpr int kCells = 9; };
+template <> struct TileOf<3, 1> { static constexpr int kCells = 4; };
+template <> struct TileOf<3, 2> { static constexpr int kCells = 2; };
+template <> struct TileOf<4, 0> { static constexpr int kCells = 4; };
+template <> struct TileOf<4, 1> { static constexpr int kCells = 2; };
+template <> struct TileOf<4, 2> { static constexpr int kCells = 1; };
+
+constexpr int MaxLds(int a, int b) { return a > b ? a : b; }
+constexpr int kBatchLds = MaxLds(MaxLds(MaxLds(Shape<2, 32>::kLdsDoubles, Shape<2, 16>::kLdsDoubles),
+                                        MaxLds(Shape<3, 9>::kLdsDoubles, Shape<3, 4>::kLdsDoubles)),
+                                 MaxLds(Shape<4, 4>::kLdsDoubles, Shape<4, 2>::kLdsDoubles));
+
+template <int Q, int kRiemann, bool kMixed>
+__device__ __forceinline__ void BatchItem(const FrHexBatchGroup &g, int64_t block, double *lds) {
+  switch (g.tile) {
+  case 0: FrHexStageBody<Q, TileOf<Q, 0>::kCells, kRiemann, true, kMixed>(g.args, g.lt, block, lds); break;
+  case 1: FrHexStageBody<Q, TileOf<Q, 1>::kCells, kRiemann, true, kMixed>(g.args, g.lt, block, lds); break;
+  default: FrHexStageBody<Q, TileOf<Q, 2>::kCells, kRiemann, true, kMixed>(g.args, g.lt, block, lds); break;
+  }
+}
+
+// Persistent batched pass: every workgroup walks the work list (blocks of all
+// degree groups, interleaved by stride), one RK stage for every case.
+template <int kRiemann, bool kMixed>
+__global__ __launch_bounds__(256) void FrHexBatchKernel(FrHexBatchArgs b) {
+  __shared__ double lds[kBatchLds];
+  for (int64_t item = blockIdx.x; item < b.n_items; item += gridDim.x) {
+    int gi = 0;
+    while (gi + 1 < b.n_groups && item >= b.group[gi + 1].first_item) ++gi;
+    const FrHexBatchGroup &g = b.group[gi];
+    const int64_t block = item - g.first_item;
+    switch (g.q) {
+    case 2: BatchItem<2, kRiemann, kMixed>(g, block, lds); break;
+    case 3: BatchItem<3, kRiemann, kMixed>(g, block, lds); break;
+    default: BatchItem<4, kRiemann, kMixed>(g, block, lds); break;
+    }
+    __syncthreads();  // the next item reuses the LDS buffers
+  }
+}
+
 template <int Q>
 __global__ __launch_bounds__(256) void FrHexTimeStepKernel(const double *u, const int32_t *geo_id,
     const double *geo, int64_t n_cells, unsigned long long *dt_min_bits) {
@@ -419,34 +477,33 @@ __global__ __launch_bounds__(256) void FrHexTimeStepKernel(const double *u, cons
 }
 
 template <int Q, int R>
-hipError_t Launch(const FrHexArgs &args, const LineTables &lt, const BcTable &bc, hipStream_t stream) {
+hipError_t Launch(const FrHexArgs &args, const LineTables &lt, hipStream_t stream) {
   constexpr int C = Shape<Q>::kCells;
   int64_t blocks = (args.n_cells + C - 1) / C;
   if (blocks > 0x7fffffff) return hipErrorInvalidValue;
   const dim3 grid((unsigned)blocks), block(256);
   if (args.mixed) {
     if (args.diag_metric) {
-      hipLaunchKernelGGL((FrHexStageKernel<Q, R, true, true>), grid, block, 0, stream, args, lt, bc);
+      hipLaunchKernelGGL((FrHexStageKernel<Q, C, R, true, true>), grid, block, 0, stream, args, lt);
     } else {
-      hipLaunchKernelGGL((FrHexStageKernel<Q, R, false, true>), grid, block, 0, stream, args, lt, bc);
+      hipLaunchKernelGGL((FrHexStageKernel<Q, C, R, false, true>), grid, block, 0, stream, args, lt);
     }
   } else {
     if (args.diag_metric) {
-      hipLaunchKernelGGL((FrHexStageKernel<Q, R, true, false>), grid, block, 0, stream, args, lt, bc);
+      hipLaunchKernelGGL((FrHexStageKernel<Q, C, R, true, false>), grid, block, 0, stream, args, lt);
     } else {
-      hipLaunchKernelGGL((FrHexStageKernel<Q, R, false, false>), grid, block, 0, stream, args, lt, bc);
+      hipLaunchKernelGGL((FrHexStageKernel<Q, C, R, false, false>), grid, block, 0, stream, args, lt);
     }
   }
   return hipGetLastError();
 }
 
 template <int Q>
-hipError_t LaunchQ(int riemann, const FrHexArgs &args, const LineTables &lt, const BcTable &bc,
-                   hipStream_t stream) {
+hipError_t LaunchQ(int riemann, const FrHexArgs &args, const LineTables &lt, hipStream_t stream) {
   switch (riemann) {
-  case kExact: return Launch<Q, kExact>(args, lt, bc, stream);
-  case kHllc: return Launch<Q, kHllc>(args, lt, bc, stream);
-  case kRoe: return Launch<Q, kRoe>(args, lt, bc, stream);
+  case kExact: return Launch<Q, kExact>(args, lt, stream);
+  case kHllc: return Launch<Q, kHllc>(args, lt, stream);
+  case kRoe: return Launch<Q, kRoe>(args, lt, stream);
   default: return hipErrorInvalidValue;
   }
 }
@@ -454,15 +511,40 @@ hipError_t LaunchQ(int riemann, const FrHexArgs &args, const LineTables &lt, con
 }  // namespace
 
 hipError_t LaunchFrHexStage(int q, int riemann, const FrHexArgs &args, const LineTables &lt,
-                            const BcTable &bc, hipStream_t stream) {
+                            hipStream_t stream) {
   switch (q) {
-  case 2: return LaunchQ<2>(riemann, args, lt, bc, stream);
-  case 3: return LaunchQ<3>(riemann, args, lt, bc, stream);
-  case 4: return LaunchQ<4>(riemann, args, lt, bc, stream);
-  case 5: return LaunchQ<5>(riemann, args, lt, bc, stream);
-  case 6: return LaunchQ<6>(riemann, args, lt, bc, stream);
+  case 2: return LaunchQ<2>(riemann, args, lt, stream);
+  case 3: return LaunchQ<3>(riemann, args, lt, stream);
+  case 4: return LaunchQ<4>(riemann, args, lt, stream);
+  case 5: return LaunchQ<5>(riemann, args, lt, stream);
+  case 6: return LaunchQ<6>(riemann, args, lt, stream);
   default: return hipErrorInvalidValue;
   }
+}
+
+int FrHexTileCells(int q, int tile) {
+  static const int cells[3][3] = { { 32, 16, 8 }, { 9, 4, 2 }, { 4, 2, 1 } };
+  if (q < 2 || q > 4 || tile < 0 || tile > 2) return 0;
+  return cells[q - 2][tile];
+}
+
+hipError_t LaunchFrHexBatch(int riemann, const FrHexBatchArgs &args, int n_workgroups, hipStream_t stream) {
+  if (args.n_groups < 1 || args.n_groups > kMaxBatchGroups) return hipErrorInvalidValue;
+  for (int g = 0; g < args.n_groups; ++g)
+    if (args.group[g].q < 2 || args.group[g].q > 4 || !args.group[g].args.diag_metric) return hipErrorInvalidValue;
+  const bool mixed = args.group[0].args.mixed != 0;
+  const dim3 grid((unsigned)n_workgroups), block(256);
+  switch (riemann) {
+#define MCFD_BATCH_CASE(R)                                                                      \
+  case R:                                                                                       \
+    if (mixed) hipLaunchKernelGGL((FrHexBatchKernel<R, true>), grid, block, 0, stream, args);   \
+    else hipLaunchKernelGGL((FrHexBatchKernel<R, false>), grid, block, 0, stream, args);        \
+    break;
+  MCFD_BATCH_CASE(kExact) MCFD_BATCH_CASE(kHllc) MCFD_BATCH_CASE(kRoe)
+#undef MCFD_BATCH_CASE
+  default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 hipError_t LaunchFrHexTimeStep(int q, const double *u, const int32_t *geo_id, const double *geo,

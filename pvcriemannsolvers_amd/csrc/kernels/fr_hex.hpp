@@ -9,7 +9,7 @@
 namespace mcfd {
 
 constexpr int kMaxQ = 6;
-constexpr int kMaxBcs = 16;
+constexpr int kMaxBatchGroups = 4;
 
 // 1-D tables of the solution-point line (Q points):
 //   d[i][m]   = l_m'(x_i)          (Lagrange derivative matrix)
@@ -19,11 +19,6 @@ struct LineTables {
   double d[kMaxQ][kMaxQ];
   double lend[2][kMaxQ];
   double gp[2][kMaxQ];
-};
-
-struct BcTable {
-  int32_t kind[kMaxBcs];
-  double state[kMaxBcs][5];
 };
 
 // Per-cell affine geometry record (20 doubles):
@@ -43,6 +38,10 @@ struct FrHexArgs {
   const int32_t *geo_id; // [cell]
   const double *geo;     // [geo][kGeoStride]
   const double *frames;  // [frame][kFrameStride]
+  const int32_t *bc_kind;   // [tag]: 0 smart, 1 inviscid wall, 2 supersonic outlet, 3 inlet
+  const double *bc_state;   // [tag][5] conservative state
+  const int32_t *cell_case; // [cell] case index, or null (one time step for all cells)
+  const double *case_dt;    // [case] time step when cell_case is set
   int64_t n_cells;
   double dt;
   int32_t stage;         // 0 residual, 1..3 SSP-RK3 stage
@@ -54,7 +53,28 @@ struct FrHexArgs {
 // Launches one residual / RK-stage pass for Q points per direction and the
 // given Riemann solver on `stream`.  Returns hipSuccess or the launch error.
 hipError_t LaunchFrHexStage(int q, int riemann, const FrHexArgs &args, const LineTables &lt,
-                            const BcTable &bc, hipStream_t stream);
+                            hipStream_t stream);
+
+// One degree group of a batched launch: its own arrays, line tables and tile
+// variant (cells per workgroup, see FrHexTileCells).
+struct FrHexBatchGroup {
+  FrHexArgs args;
+  LineTables lt;
+  int32_t q;
+  int32_t tile;
+  int64_t first_item;   // first work item of this group
+};
+struct FrHexBatchArgs {
+  FrHexBatchGroup group[kMaxBatchGroups];
+  int32_t n_groups;
+  int64_t n_items;
+};
+
+// Cells per workgroup of tile variant `tile` (0..2) for q points (2..4), 0 if invalid.
+int FrHexTileCells(int q, int tile);
+
+// Persistent batched stage over all groups with n_workgroups workgroups.
+hipError_t LaunchFrHexBatch(int riemann, const FrHexBatchArgs &args, int n_workgroups, hipStream_t stream);
 
 // Per-cell minimum of length / max wave speed over the solution points
 // (spatial/fem.hpp:417-435), reduced to one double in *dt_min (device).

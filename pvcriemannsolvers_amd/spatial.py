@@ -15,6 +15,21 @@ from . import _capi as C
 from .temporal import System
 
 
+def _mesh_desc(mesh: dict, keep: list) -> "C.MeshDesc":
+    arrs = [np.ascontiguousarray(mesh["coords"], np.float64), np.ascontiguousarray(mesh["cells"], np.int64),
+            np.ascontiguousarray(mesh["bface_nodes"], np.int64), np.ascontiguousarray(mesh["bface_tags"], np.int32)]
+    keep.extend(arrs)
+    coords, cells, bn, bt = arrs
+    md = C.MeshDesc()
+    md.cell_type = int(mesh.get("cell_type", C.HEXA_8))
+    md.n_nodes, md.coords = len(coords), C.ptr(coords, C._dp)
+    md.n_cells, md.cell_nodes = len(cells), C.ptr(cells, C._i64p)
+    md.n_bfaces, md.bface_nodes, md.bface_tags = len(bn), C.ptr(bn, C._i64p), C.ptr(bt, C._i32p)
+    for d in range(3):
+        md.period[d] = float(mesh["period"][d])
+    return md
+
+
 class FiniteElement(System):
     """A DG/FR spatial discretisation on one GPU (fem.hpp:47-436).
 
@@ -137,3 +152,97 @@ class FiniteElement(System):
                                     C.ptr(hf, C._i32p), C.ptr(sf, C._i32p), C.ptr(ijk, C._i32p)))
         return {"n_interior": ni.value, "holder": holder, "sharer": sharer, "holder_face": hf,
                 "sharer_face": sf, "fp_ijk": ijk}
+
+
+class Batch:
+    """Many independent cases advanced by one persistent launch per RK stage.
+
+    ``cases``: list of (mesh, degree, bcs) with bcs as in FiniteElement.  The
+    three stages of a step run as a captured hipGraph (``rk3_steps``).
+    """
+
+    def __init__(self, cases, riemann="hllc", nodes="legendre", c_next=1.0, precision="fp64"):
+        lib = C.load()
+        self._keep = []
+        n = len(cases)
+        descs = (C.MeshDesc * n)()
+        degrees = np.array([c[1] for c in cases], dtype=np.int32)
+        offsets = np.zeros(n + 1, dtype=np.int32)
+        all_bcs = []
+        for i, (mesh, degree, bcs) in enumerate(cases):
+            descs[i] = _mesh_desc(mesh, self._keep)
+            offsets[i + 1] = offsets[i] + len(bcs)
+            all_bcs.extend(bcs)
+        bc_arr = (C.BcDesc * max(1, len(all_bcs)))()
+        for i, (kind, state) in enumerate(all_bcs):
+            bc_arr[i].kind = FiniteElement._BC[kind]
+            for k in range(5):
+                bc_arr[i].state[k] = float(state[k]) if state is not None else 0.0
+        sd = C.SchemeDesc()
+        sd.scheme = C.SCHEME_FR_GENERAL
+        sd.degree = 0
+        sd.nodes = C.NODES_LOBATTO if nodes == "lobatto" else C.NODES_LEGENDRE
+        sd.riemann = FiniteElement._RIEMANN[riemann]
+        sd.precision = C.PRECISION_MIXED if precision == "mixed" else C.PRECISION_FP64
+        sd.c_next = c_next
+        h = ctypes.c_void_p()
+        C.check(lib.mcfd_batch_create(n, descs, C.ptr(degrees, C._i32p), ctypes.byref(sd), bc_arr,
+                                      C.ptr(offsets, C._i32p), ctypes.byref(h)))
+        self._h = h
+        self.n_cases = n
+        self.sizes = []
+        for i in range(n):
+            nc, npc = ctypes.c_int64(), ctypes.c_int32()
+            C.check(lib.mcfd_batch_case_sizes(h, i, ctypes.byref(nc), ctypes.byref(npc)))
+            self.sizes.append((nc.value, npc.value))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            C.load().mcfd_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def sample_points(self, case):
+        nc, npc = self.sizes[case]
+        out = np.empty((nc, npc, 3))
+        C.check(C.load().mcfd_batch_sample_points(self._h, case, C.ptr(out, C._dp)))
+        return out
+
+    def set_solution_column(self, case, column):
+        column = np.ascontiguousarray(column, np.float64).reshape(-1)
+        nc, npc = self.sizes[case]
+        assert column.size == nc * npc * 5
+        C.check(C.load().mcfd_batch_set_solution_column(self._h, case, C.ptr(column, C._dp)))
+
+    def get_solution_column(self, case):
+        nc, npc = self.sizes[case]
+        out = np.empty(nc * npc * 5)
+        C.check(C.load().mcfd_batch_get_solution_column(self._h, case, C.ptr(out, C._dp)))
+        return out
+
+    def get_time_steps(self, cfl):
+        out = np.empty(self.n_cases)
+        C.check(C.load().mcfd_batch_get_time_steps(self._h, float(cfl), C.ptr(out, C._dp)))
+        return out
+
+    def set_time_steps(self, dt):
+        dt = np.ascontiguousarray(dt, np.float64)
+        C.check(C.load().mcfd_batch_set_time_steps(self._h, C.ptr(dt, C._dp)))
+
+    def autotune(self, repeats=3):
+        tiles = np.zeros(4, dtype=np.int32)
+        C.check(C.load().mcfd_batch_autotune(self._h, int(repeats), C.ptr(tiles, C._i32p)))
+        return tiles
+
+    def rk3_steps(self, n_steps):
+        C.check(C.load().mcfd_batch_rk3_steps(self._h, int(n_steps)))
+
+    def synchronize(self):
+        C.check(C.load().mcfd_batch_synchronize(self._h))
+
+    def time_rk3(self, n_steps):
+        ms = ctypes.c_double()
+        C.check(C.load().mcfd_batch_time_rk3(self._h, int(n_steps), ctypes.byref(ms)))
+        return ms.value
