@@ -86,6 +86,132 @@ def cpu_baseline(n: int, steps: int, degree: int) -> dict:
                       f"{steps} steps = {3 * steps} stages in {sec:.1f} s, single thread"}
 
 
+def l2_linf(a, b, w):
+    """L2 (sqrt of sum of w_q du^2, the mesh/part.hpp:1022-1036 pattern) and Linf of rho and E."""
+    out = {}
+    for k, name in ((0, "rho"), (4, "E")):
+        d = a[..., k] - b[..., k]
+        out["l2_" + name] = float(np.sqrt((w * d * d).sum()))
+        out["linf_" + name] = float(np.abs(d).max())
+    return out
+
+
+def gl_weights(q: int, h: float) -> np.ndarray:
+    x1, w1 = np.polynomial.legendre.leggauss(q)
+    return np.einsum("i,j,k->ijk", w1, w1, w1).reshape(-1) * (h / 2) ** 3
+
+
+def bench_mixed(n: int = 48, degree: int = 4, steps: int = 100) -> dict:
+    """Mixed vs FP64 FR on the smooth periodic problem (48^3, p = 4, Roe)."""
+    from pvcriemannsolvers_amd import meshgen, spatial
+    mesh = meshgen.box_hex(n, n, n, periodic=(True, True, True))
+    states, times = {}, {}
+    for prec in ("fp64", "mixed"):
+        g = spatial.FiniteElement(mesh, degree=degree, riemann="roe", precision=prec)
+        x = g.sample_points()
+        u0 = initial_state(x)
+        del x
+        g.set_solution_column(u0.reshape(-1))
+        dt = g.get_time_step(0.1)
+        for s in range(steps):
+            g.rk3_update(s * dt, dt)
+        g.synchronize()
+        states[prec] = g.get_solution_column().reshape(u0.shape)
+        _, kern = g.time_rk3(dt, 10)
+        times[prec] = kern / 30
+        g.close()
+    w = gl_weights(degree + 1, 1.0 / n)[None, :]
+    dofs = n ** 3 * (degree + 1) ** 3
+    out = {"workload": f"FR p={degree} Roe periodic {n}^3, {steps} SSP-RK3 steps at CFL 0.1",
+           "fp64_dof_updates_per_s": dofs / (times["fp64"] * 1e-3),
+           "mixed_dof_updates_per_s": dofs / (times["mixed"] * 1e-3),
+           "fp64_stage_ms": times["fp64"], "mixed_stage_ms": times["mixed"]}
+    out.update({"mixed_vs_fp64_" + k: v for k, v in l2_linf(states["mixed"], states["fp64"], w).items()})
+    return out
+
+
+def tube_cases(n_cases: int = 256, seed: int = 7):
+    """Seeded 1-D-in-3-D shock tubes: N x 2 x 2 cubic hexahedra of edge 1/N
+    on [0,1]x[0,2/N]^2 (cubic cells keep the reference's length scale
+    cbrt(V)/2 a stable CFL measure), N ~ U{100..2000}, p ~ U{1,2,3};
+    rho ~ U[0.1,2], u ~ U[-1,1], p ~ U[0.1,10] per side; smart boundaries with
+    the side states at the x ends, inviscid walls elsewhere; the jump is
+    smoothed over two cells (tanh)."""
+    from pvcriemannsolvers_amd import meshgen
+    rng = np.random.default_rng(seed)
+    cases = []
+    for _ in range(n_cases):
+        n = int(rng.integers(100, 2001))
+        p = int(rng.integers(1, 4))
+        rho, u, pr = rng.uniform(0.1, 2, 2), rng.uniform(-1, 1, 2), rng.uniform(0.1, 10, 2)
+        sides = []
+        for i in range(2):
+            sides.append(np.array([rho[i], rho[i] * u[i], 0, 0, pr[i] / (GAMMA - 1) + 0.5 * rho[i] * u[i] ** 2]))
+        mesh = meshgen.box_hex(n, 2, 2, hi=(1.0, 2.0 / n, 2.0 / n))
+        bcs = [("smart", sides[0]), ("smart", sides[1])] + [("inviscid_wall", None)] * 4
+        cases.append((mesh, p, bcs, sides, n))
+    return cases
+
+
+def bench_batched(n_cases: int = 256, steps: int = 20) -> dict:
+    """256 shock tubes: batched (persistent launch + graph) in FP64 and mixed,
+    against one plain FP64 solver per case."""
+    from pvcriemannsolvers_amd import spatial
+    cases = tube_cases(n_cases)
+    spec = [(m, p, bcs) for m, p, bcs, _, _ in cases]
+    out = {"workload": f"{n_cases} seeded shock tubes N x 2 x 2 (cubic cells), N in [100, 2000], p in {{1,2,3}}, HLLC, "
+                       f"SSP-RK3 {steps} steps, per-case dt at CFL 0.1"}
+    finals = {}
+    for prec in ("fp64", "mixed"):
+        b = spatial.Batch(spec, riemann="hllc", precision=prec)
+        dofs = 0
+        for i, (m, p, bcs, sides, n) in enumerate(cases):
+            x = b.sample_points(i)
+            w = 0.5 * (1 + np.tanh((x[..., 0] - 0.5) * n / 2.0))
+            b.set_solution_column(i, (1 - w)[..., None] * sides[0] + w[..., None] * sides[1])
+            dofs += x.shape[0] * x.shape[1]
+        dt = b.get_time_steps(0.1)
+        b.set_time_steps(dt)
+        tiles, mode = b.autotune(repeats=3)
+        b.rk3_steps(steps)
+        b.synchronize()
+        finals[prec] = [b.get_solution_column(i) for i in range(n_cases)]
+        ms = b.time_rk3(10) / 10
+        out[prec + "_batched_dof_updates_per_s"] = 3 * dofs / (ms * 1e-3)
+        out[prec + "_batched_ms_per_step"] = ms
+        out[prec + "_tiles"] = [int(t) for t in tiles if t]
+        out[prec + "_launch"] = "persistent" if mode == 1 else "per degree group"
+        out["dofs"] = dofs
+        out["cells"] = int(sum(m["cells"].shape[0] for m, _, _, _, _ in cases))
+        b.close()
+    errs = []
+    for i, (m, p, bcs, sides, n) in enumerate(cases):
+        a = finals["mixed"][i].reshape(-1, (p + 1) ** 3, 5)
+        r = finals["fp64"][i].reshape(-1, (p + 1) ** 3, 5)
+        w = gl_weights(p + 1, 1.0 / n)[None, :]
+        errs.append(l2_linf(a, r, w)["l2_rho"])
+    out["mixed_vs_fp64_l2_rho_max"] = float(np.max(errs))
+    out["mixed_vs_fp64_l2_rho_median"] = float(np.median(errs))
+    out["all_finite"] = bool(all(np.isfinite(f).all() for f in finals["fp64"]))
+    # plain FP64: one solver per case, three launches per step and case
+    plain = [spatial.FiniteElement(m, degree=p, riemann="hllc", bcs=bcs) for m, p, bcs, _, _ in cases[:64]]
+    for g in plain:
+        g.set_solution_column(np.zeros(g.column_size) + 1.0)
+    t0 = time.perf_counter()
+    for g in plain:
+        for _ in range(3):
+            g.rk3_update(0.0, 1e-9)
+    for g in plain:
+        g.synchronize()
+    sec = (time.perf_counter() - t0) / 3
+    plain_dofs = sum(g.n_cells * g.n_per_cell for g in plain)
+    out["fp64_plain_dof_updates_per_s"] = 3 * plain_dofs / sec
+    out["plain_sample"] = "first 64 cases, one solver each, 3 steps"
+    for g in plain:
+        g.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +222,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-n", type=int, default=32)
     ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-forms", action="store_true", help="skip the mixed and batched comparisons")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -139,6 +266,13 @@ def main():
                 "kernel": "FrHexStageKernel<%d,2>" % (p + 1),
                 "avg_kernel_ms": round(avg_kernel_s * 1e3, 4),
                 "algorithmic_bytes_per_launch": int(avg_bytes)}
+    forms = None
+    g.close()
+    if not args.no_forms:
+        print("[bench] mixed vs FP64 (48^3, p=4)", file=sys.stderr, flush=True)
+        forms = {"mixed": bench_mixed()}
+        print("[bench] batched shock tubes", file=sys.stderr, flush=True)
+        forms["batched"] = bench_batched()
     cpu = None
     if not args.no_cpu:
         print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
@@ -154,6 +288,7 @@ def main():
                    "event_ms_per_step": total_ms / args.steps},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "forms": forms,
     }
     print(json.dumps(line), flush=True)
 

@@ -194,11 +194,12 @@ int mcfd_batch_get_solution_column(mcfd_batch *b, int32_t case_id, double *colum
 int mcfd_batch_get_time_steps(mcfd_batch *b, double cfl, double *dt_per_case);
 /* Time step of every case used by the following steps. */
 int mcfd_batch_set_time_steps(mcfd_batch *b, const double *dt_per_case);
-/* Times every tile variant (cells per workgroup) of every degree group with
- * `repeats` stage launches, keeps the fastest and re-captures the step
- * graph; tiles[g] = 16 * Q + variant for each degree group in ascending
- * degree (may be null). */
-int mcfd_batch_autotune(mcfd_batch *b, int32_t repeats, int32_t *tiles);
+/* Autotuner: times every tile variant (cells per workgroup) of every degree
+ * group in the persistent launch, then the persistent single launch against
+ * one launch per degree group, keeps the fastest and re-captures the step
+ * graph.  tiles[g] = 16 * Q + variant for each degree group in ascending
+ * degree; *mode = 1 persistent, 0 per group (either output may be null). */
+int mcfd_batch_autotune(mcfd_batch *b, int32_t repeats, int32_t *tiles, int32_t *mode);
 /* n_steps replays of the captured SSP-RK3 step (stream-ordered). */
 int mcfd_batch_rk3_steps(mcfd_batch *b, int32_t n_steps);
 int mcfd_batch_synchronize(mcfd_batch *b);

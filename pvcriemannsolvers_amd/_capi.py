@@ -72,7 +72,7 @@ SIGNATURES = {
     "mcfd_batch_get_solution_column": [ctypes.c_void_p, ctypes.c_int32, _dp],
     "mcfd_batch_get_time_steps": [ctypes.c_void_p, ctypes.c_double, _dp],
     "mcfd_batch_set_time_steps": [ctypes.c_void_p, _dp],
-    "mcfd_batch_autotune": [ctypes.c_void_p, ctypes.c_int32, _i32p],
+    "mcfd_batch_autotune": [ctypes.c_void_p, ctypes.c_int32, _i32p, _i32p],
     "mcfd_batch_rk3_steps": [ctypes.c_void_p, ctypes.c_int32],
     "mcfd_batch_synchronize": [ctypes.c_void_p],
     "mcfd_batch_time_rk3": [ctypes.c_void_p, ctypes.c_int32, _dp],

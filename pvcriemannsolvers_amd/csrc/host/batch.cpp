@@ -19,7 +19,7 @@ using mcfd::Guard;
 
 namespace {
 
-constexpr int kPersistentWorkgroups = 1024;  // 256 CUs x 4 resident workgroups
+constexpr int kPersistentWorkgroups = 2048;  // 256 CUs x 8 resident workgroups
 
 struct CaseRef {
   int group;
@@ -41,13 +41,22 @@ struct mcfd_batch {
   hipGraphExec_t exec = nullptr;
   int riemann = 0;
 
-  mcfd::FrHexBatchArgs Args(int stage, int only_group = -1, int tile = -1) const {
-    mcfd::FrHexBatchArgs b;
-    std::memset(&b, 0, sizeof(b));
+  bool mixed = false;
+  // 0: one launch per degree group and stage; 1: one persistent launch per
+  // stage over all groups.  Chosen by mcfd_batch_autotune.
+  int mode = 1;
+  // Group tables of the three stages (device) and of autotuning trials.
+  mcfd::DeviceArray<mcfd::FrHexBatchGroup> stage_tables, trial_table;
+
+  std::vector<mcfd::FrHexBatchGroup> Groups(int stage, int only_group, int tile, int64_t *n_items,
+                                            int *max_lds) const {
+    std::vector<mcfd::FrHexBatchGroup> out;
     int64_t items = 0;
+    *max_lds = 0;
     for (int g = 0; g < (int)groups.size(); ++g) {
       if (only_group >= 0 && g != only_group) continue;
-      auto &dst = b.group[b.n_groups++];
+      mcfd::FrHexBatchGroup dst;
+      std::memset(&dst, 0, sizeof(dst));
       const auto &grp = *groups[g];
       dst.args = grp.Args(stage, 0.0);
       dst.args.case_dt = case_dt.ptr;
@@ -57,13 +66,31 @@ struct mcfd_batch {
       dst.first_item = items;
       const int cells = mcfd::FrHexTileCells(grp.q, dst.tile);
       items += (grp.n_cells + cells - 1) / cells;
+      *max_lds = std::max(*max_lds, mcfd::FrHexTileLdsBytes(grp.q, dst.tile));
+      out.push_back(dst);
     }
-    b.n_items = items;
-    return b;
+    *n_items = items;
+    return out;
   }
-  void Launch(const mcfd::FrHexBatchArgs &args) {
-    int wg = (int)std::min<int64_t>(args.n_items, kPersistentWorkgroups);
-    Check(mcfd::LaunchFrHexBatch(riemann, args, wg, stream), "batched stage launch");
+  void Launch(const mcfd::FrHexBatchGroup *table, int n_groups, int64_t n_items, int max_lds) {
+    mcfd::FrHexBatchArgs a;
+    a.group = table;
+    a.n_groups = n_groups;
+    a.max_tile_lds = max_lds;
+    a.n_items = n_items;
+    int resident = mcfd::FrHexBatchResidentWorkgroups(riemann, mixed, max_lds);
+    if (resident < 1) resident = 1024;
+    int wg = (int)std::min<int64_t>(n_items, resident);
+    Check(mcfd::LaunchFrHexBatch(riemann, mixed, a, wg, stream), "batched stage launch");
+  }
+  // Uploads the three stage tables and returns their launch parameters.
+  void UploadStages(int64_t *n_items, int *max_lds) {
+    std::vector<mcfd::FrHexBatchGroup> all;
+    for (int stage = 1; stage <= 3; ++stage) {
+      auto t = Groups(stage, -1, -1, n_items, max_lds);
+      all.insert(all.end(), t.begin(), t.end());
+    }
+    stage_tables.Upload(all, stream);
   }
   void ReleaseGraph() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -71,11 +98,27 @@ struct mcfd_batch {
     exec = nullptr;
     graph = nullptr;
   }
-  // Captures the three stage launches of one SSP-RK3 step.
+  // One stage of every case in the current mode.
+  void Stage(int stage, int64_t n_items, int max_lds) {
+    if (mode == 1) {
+      Launch(stage_tables.ptr + (stage - 1) * groups.size(), (int)groups.size(), n_items, max_lds);
+    } else {
+      for (auto &g : groups) {
+        auto a = g->Args(stage, 0.0);
+        a.case_dt = case_dt.ptr;
+        Check(mcfd::LaunchFrHexStage(g->q, riemann, a, g->lt, stream), "group stage launch");
+      }
+    }
+  }
+  // Captures the three stages of one SSP-RK3 step.
   void Capture() {
     ReleaseGraph();
+    int64_t n_items;
+    int max_lds;
+    UploadStages(&n_items, &max_lds);
+    Check(hipStreamSynchronize(stream), "table upload");
     Check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture");
-    for (int stage = 1; stage <= 3; ++stage) Launch(Args(stage));
+    for (int stage = 1; stage <= 3; ++stage) Stage(stage, n_items, max_lds);
     Check(hipStreamEndCapture(stream, &graph), "end capture");
     Check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
   }
@@ -101,6 +144,7 @@ int mcfd_batch_create(int32_t n_cases, const mcfd_mesh_desc *meshes, const int32
     if (scheme->scheme != MCFD_SCHEME_FR_GENERAL) throw mcfd::Failure{ MCFD_ENOTSUP, "batch implements FR_GENERAL" };
     b->n_cases = n_cases;
     b->riemann = scheme->riemann;
+    b->mixed = scheme->precision == MCFD_PRECISION_MIXED;
     const int n_bcs = bc_offsets[n_cases];
     if (n_bcs < 0 || (n_bcs > 0 && !bcs)) throw std::invalid_argument("bad BC table");
     std::vector<int32_t> kinds(n_bcs);
@@ -259,30 +303,50 @@ int mcfd_batch_set_time_steps(mcfd_batch *b, const double *dt) {
   });
 }
 
-int mcfd_batch_autotune(mcfd_batch *b, int32_t repeats, int32_t *tiles) {
+int mcfd_batch_autotune(mcfd_batch *b, int32_t repeats, int32_t *tiles, int32_t *mode) {
   if (!b || repeats < 1) return Fail(MCFD_EINVAL, "bad argument");
   return Guard([&] {
     hipEvent_t e0, e1;
     Check(hipEventCreate(&e0), "event");
     Check(hipEventCreate(&e1), "event");
     struct Ev { hipEvent_t a, b; ~Ev() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); } } guard{ e0, e1 };
+    auto time = [&](auto &&launch) {
+      launch();  // warm-up
+      Check(hipEventRecord(e0, b->stream), "event");
+      for (int r = 0; r < repeats; ++r) launch();
+      Check(hipEventRecord(e1, b->stream), "event");
+      Check(hipEventSynchronize(e1), "event sync");
+      float ms;
+      Check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+      return ms;
+    };
+    // tile variant per degree group for the persistent form (stage 1 writes
+    // only the U1 scratch state)
     for (int g = 0; g < (int)b->groups.size(); ++g) {
       float best = 1e30f;
       int best_tile = 0;
       for (int tile = 0; tile < 3; ++tile) {
-        auto args = b->Args(1, g, tile);  // stage 1 writes only the U1 scratch state
-        b->Launch(args);                  // warm-up
-        Check(hipEventRecord(e0, b->stream), "event");
-        for (int r = 0; r < repeats; ++r) b->Launch(args);
-        Check(hipEventRecord(e1, b->stream), "event");
-        Check(hipEventSynchronize(e1), "event sync");
-        float ms;
-        Check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+        int64_t n_items;
+        int max_lds;
+        auto table = b->Groups(1, g, tile, &n_items, &max_lds);
+        b->trial_table.Upload(table, b->stream);
+        float ms = time([&] { b->Launch(b->trial_table.ptr, 1, n_items, max_lds); });
         if (ms < best) { best = ms; best_tile = tile; }
       }
       b->tiles[g] = best_tile;
       if (tiles) tiles[g] = b->groups[g]->q * 16 + best_tile;
     }
+    // persistent single launch vs one launch per degree group
+    int64_t n_items;
+    int max_lds;
+    b->UploadStages(&n_items, &max_lds);
+    float ms_mode[2];
+    for (int m = 0; m < 2; ++m) {
+      b->mode = m;
+      ms_mode[m] = time([&] { b->Stage(1, n_items, max_lds); });
+    }
+    b->mode = ms_mode[1] <= ms_mode[0] ? 1 : 0;
+    if (mode) *mode = b->mode;
     b->Capture();
   });
 }

@@ -23,6 +23,8 @@
 // launch that walks a work list spanning several polynomial degrees.
 #include "fr_hex.hpp"
 
+#include <algorithm>
+
 #include "euler.hpp"
 
 namespace mcfd {
@@ -417,27 +419,39 @@ template <> struct TileOf<4, 0> { static constexpr int kCells = 4; };
 template <> struct TileOf<4, 1> { static constexpr int kCells = 2; };
 template <> struct TileOf<4, 2> { static constexpr int kCells = 1; };
 
-constexpr int MaxLds(int a, int b) { return a > b ? a : b; }
-constexpr int kBatchLds = MaxLds(MaxLds(MaxLds(Shape<2, 32>::kLdsDoubles, Shape<2, 16>::kLdsDoubles),
-                                        MaxLds(Shape<3, 9>::kLdsDoubles, Shape<3, 4>::kLdsDoubles)),
-                                 MaxLds(Shape<4, 4>::kLdsDoubles, Shape<4, 2>::kLdsDoubles));
+template <int Q, int V>
+constexpr int TileLds() { return Shape<Q, TileOf<Q, V>::kCells>::kLdsDoubles; }
+
+// One out-of-line body per (Q, tile): each is register-allocated on its own
+// instead of the batched kernel carrying the union of nine inlined bodies.
+template <int Q, int V, int kRiemann, bool kMixed>
+__device__ __noinline__ void BatchBody(const FrHexBatchGroup *g, int64_t block, double *lds) {
+  FrHexStageBody<Q, TileOf<Q, V>::kCells, kRiemann, true, kMixed>(g->args, g->lt, block, lds);
+}
 
 template <int Q, int kRiemann, bool kMixed>
 __device__ __forceinline__ void BatchItem(const FrHexBatchGroup &g, int64_t block, double *lds) {
   switch (g.tile) {
-  case 0: FrHexStageBody<Q, TileOf<Q, 0>::kCells, kRiemann, true, kMixed>(g.args, g.lt, block, lds); break;
-  case 1: FrHexStageBody<Q, TileOf<Q, 1>::kCells, kRiemann, true, kMixed>(g.args, g.lt, block, lds); break;
-  default: FrHexStageBody<Q, TileOf<Q, 2>::kCells, kRiemann, true, kMixed>(g.args, g.lt, block, lds); break;
+  case 0: BatchBody<Q, 0, kRiemann, kMixed>(&g, block, lds); break;
+  case 1: BatchBody<Q, 1, kRiemann, kMixed>(&g, block, lds); break;
+  default: BatchBody<Q, 2, kRiemann, kMixed>(&g, block, lds); break;
   }
 }
 
-// Persistent batched pass: every workgroup walks the work list (blocks of all
-// degree groups, interleaved by stride), one RK stage for every case.
+// Persistent batched pass: workgroup w processes the contiguous item range
+// [w * n_items / W, (w + 1) * n_items / W) of the work list (blocks of all
+// degree groups in order), so consecutive blocks -- and their neighbour
+// cells -- stay in one XCD's L2.  LDS is sized at launch for the largest tile
+// in use.
 template <int kRiemann, bool kMixed>
 __global__ __launch_bounds__(256) void FrHexBatchKernel(FrHexBatchArgs b) {
-  __shared__ double lds[kBatchLds];
-  for (int64_t item = blockIdx.x; item < b.n_items; item += gridDim.x) {
-    int gi = 0;
+  extern __shared__ double lds[];
+  const int64_t w = blockIdx.x, nw = gridDim.x;
+  const int64_t begin = w * b.n_items / nw, end = (w + 1) * b.n_items / nw;
+  int gi = 0;
+  for (int64_t item = begin; item < end; ++item) {
+    // the group table lives in device memory and gi is uniform, so every
+    // field is fetched with scalar loads
     while (gi + 1 < b.n_groups && item >= b.group[gi + 1].first_item) ++gi;
     const FrHexBatchGroup &g = b.group[gi];
     const int64_t block = item - g.first_item;
@@ -522,29 +536,53 @@ hipError_t LaunchFrHexStage(int q, int riemann, const FrHexArgs &args, const Lin
   }
 }
 
+int FrHexTileLdsBytes(int q, int tile) {
+  static const int lds[3][3] = { { TileLds<2, 0>(), TileLds<2, 1>(), TileLds<2, 2>() },
+                                 { TileLds<3, 0>(), TileLds<3, 1>(), TileLds<3, 2>() },
+                                 { TileLds<4, 0>(), TileLds<4, 1>(), TileLds<4, 2>() } };
+  if (q < 2 || q > 4 || tile < 0 || tile > 2) return 0;
+  return lds[q - 2][tile] * (int)sizeof(double);
+}
+
 int FrHexTileCells(int q, int tile) {
   static const int cells[3][3] = { { 32, 16, 8 }, { 9, 4, 2 }, { 4, 2, 1 } };
   if (q < 2 || q > 4 || tile < 0 || tile > 2) return 0;
   return cells[q - 2][tile];
 }
 
-hipError_t LaunchFrHexBatch(int riemann, const FrHexBatchArgs &args, int n_workgroups, hipStream_t stream) {
-  if (args.n_groups < 1 || args.n_groups > kMaxBatchGroups) return hipErrorInvalidValue;
-  for (int g = 0; g < args.n_groups; ++g)
-    if (args.group[g].q < 2 || args.group[g].q > 4 || !args.group[g].args.diag_metric) return hipErrorInvalidValue;
-  const bool mixed = args.group[0].args.mixed != 0;
+hipError_t LaunchFrHexBatch(int riemann, bool mixed, const FrHexBatchArgs &args, int n_workgroups,
+                            hipStream_t stream) {
+  if (args.n_groups < 1 || args.n_groups > kMaxBatchGroups || n_workgroups < 1) return hipErrorInvalidValue;
   const dim3 grid((unsigned)n_workgroups), block(256);
+  const size_t lds = (size_t)args.max_tile_lds;
   switch (riemann) {
-#define MCFD_BATCH_CASE(R)                                                                      \
-  case R:                                                                                       \
-    if (mixed) hipLaunchKernelGGL((FrHexBatchKernel<R, true>), grid, block, 0, stream, args);   \
-    else hipLaunchKernelGGL((FrHexBatchKernel<R, false>), grid, block, 0, stream, args);        \
+#define MCFD_BATCH_CASE(R)                                                                        \
+  case R:                                                                                         \
+    if (mixed) hipLaunchKernelGGL((FrHexBatchKernel<R, true>), grid, block, lds, stream, args);   \
+    else hipLaunchKernelGGL((FrHexBatchKernel<R, false>), grid, block, lds, stream, args);        \
     break;
   MCFD_BATCH_CASE(kExact) MCFD_BATCH_CASE(kHllc) MCFD_BATCH_CASE(kRoe)
 #undef MCFD_BATCH_CASE
   default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+int FrHexBatchResidentWorkgroups(int riemann, bool mixed, int lds_bytes) {
+  int per_cu = 0;
+  hipError_t e = hipErrorInvalidValue;
+  switch (riemann * 2 + (mixed ? 1 : 0)) {
+#define MCFD_OCC(R, M) \
+  case R * 2 + M: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, FrHexBatchKernel<R, M>, 256, lds_bytes); break;
+  MCFD_OCC(kExact, 0) MCFD_OCC(kExact, 1) MCFD_OCC(kHllc, 0) MCFD_OCC(kHllc, 1) MCFD_OCC(kRoe, 0) MCFD_OCC(kRoe, 1)
+#undef MCFD_OCC
+  default: break;
+  }
+  int dev = 0, cus = 0;
+  if (e != hipSuccess || hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 0;
+  return std::max(1, per_cu) * cus;
 }
 
 hipError_t LaunchFrHexTimeStep(int q, const double *u, const int32_t *geo_id, const double *geo,

@@ -65,16 +65,23 @@ struct FrHexBatchGroup {
   int64_t first_item;   // first work item of this group
 };
 struct FrHexBatchArgs {
-  FrHexBatchGroup group[kMaxBatchGroups];
+  const FrHexBatchGroup *group;  // device array of n_groups entries (one stage)
   int32_t n_groups;
+  int32_t max_tile_lds;          // LDS bytes of the largest tile in use
   int64_t n_items;
 };
 
-// Cells per workgroup of tile variant `tile` (0..2) for q points (2..4), 0 if invalid.
+// Cells per workgroup and LDS bytes of tile variant `tile` (0..2) for q
+// points (2..4); 0 if invalid.
 int FrHexTileCells(int q, int tile);
+int FrHexTileLdsBytes(int q, int tile);
 
 // Persistent batched stage over all groups with n_workgroups workgroups.
-hipError_t LaunchFrHexBatch(int riemann, const FrHexBatchArgs &args, int n_workgroups, hipStream_t stream);
+hipError_t LaunchFrHexBatch(int riemann, bool mixed, const FrHexBatchArgs &args, int n_workgroups,
+                            hipStream_t stream);
+// Workgroups of the batched kernel resident on the whole device with the
+// given dynamic LDS (0 on failure).
+int FrHexBatchResidentWorkgroups(int riemann, bool mixed, int lds_bytes);
 
 // Per-cell minimum of length / max wave speed over the solution points
 // (spatial/fem.hpp:417-435), reduced to one double in *dt_min (device).

@@ -232,9 +232,11 @@ class Batch:
         C.check(C.load().mcfd_batch_set_time_steps(self._h, C.ptr(dt, C._dp)))
 
     def autotune(self, repeats=3):
+        """Returns (tiles per degree group, mode): mode 1 = persistent launch."""
         tiles = np.zeros(4, dtype=np.int32)
-        C.check(C.load().mcfd_batch_autotune(self._h, int(repeats), C.ptr(tiles, C._i32p)))
-        return tiles
+        mode = ctypes.c_int32()
+        C.check(C.load().mcfd_batch_autotune(self._h, int(repeats), C.ptr(tiles, C._i32p), ctypes.byref(mode)))
+        return tiles, mode.value
 
     def rk3_steps(self, n_steps):
         C.check(C.load().mcfd_batch_rk3_steps(self._h, int(n_steps)))
