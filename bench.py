@@ -215,8 +215,8 @@ def bench_batched(n_cases: int = 256, steps: int = 20) -> dict:
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--n", type=int, default=64, help="cells per direction")
     ap.add_argument("--degree", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
@@ -246,7 +246,7 @@ def main():
     g.synchronize()
     print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    total_ms, kernel_ms = g.time_rk3(dt, args.steps)
+    total_ms, kernel_ms, stage_ms = g.time_rk3(dt, args.steps, per_stage=True)
     g.synchronize()
     wall = time.perf_counter() - t0
     state = g.get_solution_column()
@@ -265,6 +265,8 @@ def main():
                 "traffic": read_traffic(f"fr_hex_stage_{n}cubed_p{p}_hbm_traffic.json"),
                 "kernel": "FrHexStageKernel<%d,2>" % (p + 1),
                 "avg_kernel_ms": round(avg_kernel_s * 1e3, 4),
+                "stage_ms_p10_median_p90": [round(float(v), 4) for v in np.percentile(stage_ms, [10, 50, 90])],
+                "stages_timed": int(stage_ms.size),
                 "algorithmic_bytes_per_launch": int(avg_bytes)}
     forms = None
     g.close()

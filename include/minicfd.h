@@ -163,10 +163,12 @@ int mcfd_faces(const mcfd_solver *h, int64_t *holder, int64_t *sharer,
                int32_t *holder_face, int32_t *sharer_face, int32_t *fp_ijk);
 
 /* Stream synchronisation and per-kernel timing of n RK3 steps with HIP
- * events on the handle's stream: total ms, and ms of the stage kernels. */
+ * events on the handle's stream: total ms, summed ms of the stage kernels,
+ * and (if stage_ms is not null) the ms of each of the 3 * n_steps stage
+ * kernels in launch order. */
 int mcfd_synchronize(mcfd_solver *h);
 int mcfd_time_rk3(mcfd_solver *h, double dt, int32_t n_steps,
-                  double *total_ms, double *stage_kernel_ms);
+                  double *total_ms, double *stage_kernel_ms, double *stage_ms);
 
 
 /* ---- batched form

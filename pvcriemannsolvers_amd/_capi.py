@@ -62,7 +62,7 @@ SIGNATURES = {
     "mcfd_face_count": [ctypes.c_void_p, _i64p, _i64p],
     "mcfd_faces": [ctypes.c_void_p, _i64p, _i64p, _i32p, _i32p, _i32p],
     "mcfd_synchronize": [ctypes.c_void_p],
-    "mcfd_time_rk3": [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32, _dp, _dp],
+    "mcfd_time_rk3": [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32, _dp, _dp, _dp],
     "mcfd_batch_create": [ctypes.c_int32, ctypes.POINTER(MeshDesc), _i32p, ctypes.POINTER(SchemeDesc),
                           ctypes.POINTER(BcDesc), _i32p, ctypes.POINTER(ctypes.c_void_p)],
     "mcfd_batch_destroy": [ctypes.c_void_p],

@@ -19,8 +19,6 @@ using mcfd::Guard;
 
 namespace {
 
-constexpr int kPersistentWorkgroups = 2048;  // 256 CUs x 8 resident workgroups
-
 struct CaseRef {
   int group;
   int64_t first_cell, n_cells;

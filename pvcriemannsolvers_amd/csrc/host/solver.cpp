@@ -196,7 +196,8 @@ int mcfd_synchronize(mcfd_solver *h) {
   return Guard([&] { Check(hipStreamSynchronize(h->stream), "synchronize"); });
 }
 
-int mcfd_time_rk3(mcfd_solver *h, double dt, int32_t n_steps, double *total_ms, double *stage_kernel_ms) {
+int mcfd_time_rk3(mcfd_solver *h, double dt, int32_t n_steps, double *total_ms, double *stage_kernel_ms,
+                  double *stage_ms) {
   MCFD_REQUIRE_DEVICE(h);
   if (n_steps < 1) return Fail(MCFD_EINVAL, "n_steps must be positive");
   return Guard([&] {
@@ -223,6 +224,7 @@ int mcfd_time_rk3(mcfd_solver *h, double dt, int32_t n_steps, double *total_ms, 
     for (size_t i = 2; i < ev.size(); i += 2) {
       Check(hipEventElapsedTime(&ms, ev[i], ev[i + 1]), "elapsed");
       kernel_ms += ms;
+      if (stage_ms) stage_ms[(i - 2) / 2] = ms;
     }
     Check(hipEventElapsedTime(&ms, ev[0], ev[1]), "elapsed");
     if (total_ms) *total_ms = ms;

@@ -122,11 +122,13 @@ class FiniteElement(System):
         C.check(C.load().mcfd_get_time_step(self._h, float(cfl), float(dt_guess), ctypes.byref(dt)))
         return dt.value
 
-    def time_rk3(self, dt, n_steps):
+    def time_rk3(self, dt, n_steps, per_stage=False):
+        """Event-timed fused RK3 steps: (total ms, summed stage-kernel ms[, ms of each stage])."""
         total, kern = ctypes.c_double(), ctypes.c_double()
+        stages = np.zeros(3 * int(n_steps))
         C.check(C.load().mcfd_time_rk3(self._h, float(dt), int(n_steps), ctypes.byref(total),
-                                       ctypes.byref(kern)))
-        return total.value, kern.value
+                                       ctypes.byref(kern), C.ptr(stages, C._dp)))
+        return (total.value, kern.value, stages) if per_stage else (total.value, kern.value)
 
     # ---- set-up helpers
     def sample_points(self):
