@@ -1,6 +1,7 @@
 // FrHexGroup: host construction and device upload of one FR mesh.
 #include "fr_group.hpp"
 
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -149,6 +150,44 @@ void FrHexGroup::Upload(hipStream_t stream) {
   bc_kind.Upload(h_bc_kind, stream);
   bc_state.Upload(h_bc_state, stream);
   if (!h_cell_case.empty()) cell_case.Upload(h_cell_case, stream);
+  if (!h_tile_order.empty()) tile_order.Upload(h_tile_order, stream);
+}
+
+void FrHexGroup::BuildTileOrder(int tile_cells) {
+  h_tile_order.clear();
+  if (tile_cells < 1 || n_cells == 0) return;
+  const int64_t n_tiles = (n_cells + tile_cells - 1) / tile_cells;
+  std::vector<std::array<double, 3>> centre(n_tiles, { 0, 0, 0 });
+  double lo[3] = { 1e300, 1e300, 1e300 }, hi[3] = { -1e300, -1e300, -1e300 };
+  for (int64_t t = 0; t < n_tiles; ++t) {
+    const int64_t c0 = t * tile_cells, c1 = std::min<int64_t>(n_cells, c0 + tile_cells);
+    for (int64_t c = c0; c < c1; ++c)
+      for (int d = 0; d < 3; ++d) centre[t][d] += corner0[c][d] / double(c1 - c0);
+    for (int d = 0; d < 3; ++d) { lo[d] = std::min(lo[d], centre[t][d]); hi[d] = std::max(hi[d], centre[t][d]); }
+  }
+  auto spread = [](uint64_t v) {  // 21 bits -> every third bit
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+  };
+  std::vector<std::pair<uint64_t, int32_t>> key(n_tiles);
+  for (int64_t t = 0; t < n_tiles; ++t) {
+    uint64_t code = 0;
+    for (int d = 0; d < 3; ++d) {
+      const double span = hi[d] - lo[d];
+      const uint64_t q = span > 0 ? (uint64_t)((centre[t][d] - lo[d]) / span * 2097151.0) : 0;
+      code |= spread(q) << d;
+    }
+    key[t] = { code, (int32_t)t };
+  }
+  std::stable_sort(key.begin(), key.end(),
+                   [](const auto &a, const auto &b) { return a.first < b.first; });
+  h_tile_order.resize(n_tiles);
+  for (int64_t t = 0; t < n_tiles; ++t) h_tile_order[t] = key[t].second;
 }
 
 FrHexArgs FrHexGroup::Args(int stage, double dt) const {
@@ -163,6 +202,7 @@ FrHexArgs FrHexGroup::Args(int stage, double dt) const {
   a.dt = dt;
   a.stage = stage;
   a.xcd_remap = 1;
+  a.tile_order = tile_order.ptr;
   a.diag_metric = diag_metric ? 1 : 0;
   a.mixed = mixed ? 1 : 0;
   switch (stage) {

@@ -23,15 +23,20 @@ struct FrHexGroup {
   std::vector<double> x1d;
   std::vector<std::array<double, 3>> corner0;  // node 0 of each cell
   // host copies uploaded by Upload()
-  std::vector<int32_t> h_nbr, h_finfo, h_geo_id, h_bc_kind, h_cell_case;
+  std::vector<int32_t> h_nbr, h_finfo, h_geo_id, h_bc_kind, h_cell_case, h_tile_order;
   std::vector<double> h_geo, h_frames, h_bc_state;
   DeviceArray<double> u0, u1, u2, res;
-  DeviceArray<int32_t> nbr, finfo, geo_id, bc_kind, cell_case;
+  DeviceArray<int32_t> nbr, finfo, geo_id, bc_kind, cell_case, tile_order;
   DeviceArray<double> geo, frames, bc_state, case_dt;
 
   // Host-side construction; bc_kind/bc_state index the mesh's boundary tags.
   void Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &scheme,
              const std::vector<int32_t> &bc_kinds, const std::vector<double> &bc_states);
+  // Order in which the stage kernel sweeps its tiles of `tile_cells`
+  // consecutive cells: Morton order of the tile centres, so that the tiles in
+  // flight on one XCD are spatial neighbours and their face-trace gathers hit
+  // that XCD's L2.  Uploaded by Upload(); without it tiles go in id order.
+  void BuildTileOrder(int tile_cells);
   // Device allocation and upload on `stream` (not synchronised).
   void Upload(hipStream_t stream);
 

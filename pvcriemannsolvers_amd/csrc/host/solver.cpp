@@ -64,6 +64,7 @@ int mcfd_create(const mcfd_mesh_desc *mesh, const mcfd_scheme_desc *scheme, cons
     h->group.Build(*mesh, *scheme, kinds, states);
     h->host_only = (scheme->flags & MCFD_FLAG_HOST_ONLY) != 0;
     if (h->host_only) return;
+    h->group.BuildTileOrder(mcfd::FrHexStageTileCells(h->group.q));
     Check(hipStreamCreate(&h->stream), "hipStreamCreate");
     h->group.Upload(h->stream);
     h->scratch.Allocate(1);

@@ -403,7 +403,9 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
 template <int Q, int Cells, int kRiemann, bool kDiag, bool kMixed>
 __global__ __launch_bounds__(256) void FrHexStageKernel(FrHexArgs a, LineTables lt) {
   __shared__ double lds[Shape<Q, Cells>::kLdsDoubles];
-  FrHexStageBody<Q, Cells, kRiemann, kDiag, kMixed>(a, lt, LogicalBlock(a.xcd_remap), lds);
+  int64_t block = LogicalBlock(a.xcd_remap);
+  if (a.tile_order) block = a.tile_order[block];
+  FrHexStageBody<Q, Cells, kRiemann, kDiag, kMixed>(a, lt, block, lds);
 }
 
 // Tile variants per Q (cells per workgroup) that the batched launch and its
@@ -542,6 +544,17 @@ int FrHexTileLdsBytes(int q, int tile) {
                                  { TileLds<4, 0>(), TileLds<4, 1>(), TileLds<4, 2>() } };
   if (q < 2 || q > 4 || tile < 0 || tile > 2) return 0;
   return lds[q - 2][tile] * (int)sizeof(double);
+}
+
+int FrHexStageTileCells(int q) {
+  switch (q) {
+  case 2: return Shape<2>::kCells;
+  case 3: return Shape<3>::kCells;
+  case 4: return Shape<4>::kCells;
+  case 5: return Shape<5>::kCells;
+  case 6: return Shape<6>::kCells;
+  default: return 0;
+  }
 }
 
 int FrHexTileCells(int q, int tile) {

@@ -48,12 +48,15 @@ struct FrHexArgs {
   int32_t xcd_remap;     // remap block ids so each XCD sweeps a contiguous range
   int32_t diag_metric;   // every J* is diagonal (axis-aligned cells)
   int32_t mixed;         // FP32 flux-column contractions (mixed-precision form)
+  const int32_t *tile_order;  // [tile]: tile swept by logical block b (spatial locality), or null
 };
 
 // Launches one residual / RK-stage pass for Q points per direction and the
 // given Riemann solver on `stream`.  Returns hipSuccess or the launch error.
 hipError_t LaunchFrHexStage(int q, int riemann, const FrHexArgs &args, const LineTables &lt,
                             hipStream_t stream);
+// Cells per workgroup (one tile) of LaunchFrHexStage for Q points per direction.
+int FrHexStageTileCells(int q);
 
 // One degree group of a batched launch: its own arrays, line tables and tile
 // variant (cells per workgroup, see FrHexTileCells).
