@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "core.hpp"
+#include "dg.hpp"
 #include "fr.hpp"
 #include "mesh.hpp"
 
@@ -19,6 +20,11 @@ namespace {
 struct FrHandle {
   Mesh mesh;
   std::unique_ptr<FrScheme> scheme;
+};
+
+struct DgHandle {
+  Mesh mesh;
+  std::unique_ptr<DgScheme> scheme;
 };
 
 thread_local std::string g_error;
@@ -220,6 +226,90 @@ double oracle_fr_time_residuals(void *h, double const *u, double *r, int n) {
   auto t0 = std::chrono::steady_clock::now();
   for (int i = 0; i < n; ++i) static_cast<FrHandle *>(h)->scheme->Residual(u, r);
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+
+// ---------------------------------------------------------------- modal DG (hexahedra, tetrahedra)
+int oracle_dg_create(int cell_type, int64_t n_nodes, double const *coords, int64_t n_cells,
+    int64_t const *cells, int64_t n_bfaces, int64_t const *bface_nodes, int32_t const *bface_tags,
+    double const *period, int degree, int riemann, int n_bc, int32_t const *bc_kinds,
+    double const *bc_states, void **out) {
+  return Guard([&] {
+    auto h = std::make_unique<DgHandle>();
+    h->mesh = BuildMesh(MakeInput(cell_type, n_nodes, coords, n_cells, cells, n_bfaces, bface_nodes,
+        bface_tags, period), degree + 1, false);
+    h->scheme = std::make_unique<DgScheme>(&h->mesh, degree, riemann, MakeBcs(n_bc, bc_kinds, bc_states));
+    *out = h.release();
+  });
+}
+void oracle_dg_destroy(void *h) { delete static_cast<DgHandle *>(h); }
+// n_cells, basis functions per cell, quadrature points per cell, faces, interior faces
+void oracle_dg_sizes(void *h, int64_t *out5) {
+  auto *p = static_cast<DgHandle *>(h);
+  out5[0] = p->mesh.cells.size();
+  out5[1] = p->scheme->n;
+  out5[2] = p->mesh.cells.empty() ? 0 : p->mesh.cells[0].qp.size();
+  out5[3] = p->mesh.faces.size();
+  out5[4] = p->mesh.n_interior;
+}
+// Cell quadrature points [cell][q][3], global weights [cell][q], centres [cell][3],
+// Gram-Schmidt factors [cell][N][N] and lengths [cell].
+void oracle_dg_cells(void *h, double *xyz, double *w, double *centers, double *s, double *lengths) {
+  auto *p = static_cast<DgHandle *>(h);
+  int n = p->scheme->n;
+  size_t iq = 0;
+  for (size_t c = 0; c < p->mesh.cells.size(); ++c) {
+    auto const &cell = p->mesh.cells[c];
+    for (size_t q = 0; q < cell.qp.size(); ++q, ++iq) {
+      for (int d = 0; d < 3; ++d) xyz[3 * iq + d] = cell.qp[q][d];
+      w[iq] = cell.qw[q];
+    }
+    for (int d = 0; d < 3; ++d) centers[3 * c + d] = p->scheme->basis[c].center[d];
+    for (int i = 0; i < n; ++i) for (int j = 0; j < n; ++j) s[(c * n + i) * n + j] = p->scheme->basis[c].s[i][j];
+    lengths[c] = cell.length;
+  }
+}
+// Faces: holder, sharer, local face ids, tag; per face point the global
+// point [3], weight and frame [9] (face points per face = out of sizes()).
+void oracle_dg_faces(void *h, int64_t *holder, int64_t *sharer, int32_t *holder_face, int32_t *sharer_face,
+    int32_t *tag, double *xyz, double *w, double *frames) {
+  auto *p = static_cast<DgHandle *>(h);
+  size_t iq = 0;
+  for (size_t f = 0; f < p->mesh.faces.size(); ++f) {
+    auto const &face = p->mesh.faces[f];
+    holder[f] = face.holder; sharer[f] = face.sharer;
+    holder_face[f] = face.holder_face; sharer_face[f] = face.sharer_face; tag[f] = face.tag;
+    for (size_t q = 0; q < face.qp.size(); ++q, ++iq) {
+      for (int d = 0; d < 3; ++d) xyz[3 * iq + d] = face.qp[q][d];
+      w[iq] = face.qw[q];
+      auto const &fr = face.frames[q];
+      for (int d = 0; d < 3; ++d) {
+        frames[9 * iq + d] = fr.a[d]; frames[9 * iq + 3 + d] = fr.b[d]; frames[9 * iq + 6 + d] = fr.c[d];
+      }
+    }
+  }
+}
+int oracle_dg_residual(void *h, double const *u, double *r) {
+  return Guard([&] { static_cast<DgHandle *>(h)->scheme->Residual(u, r); });
+}
+int oracle_dg_rk3(void *h, double *u, double dt, int n_steps) {
+  return Guard([&] {
+    auto *p = static_cast<DgHandle *>(h);
+    size_t size = p->mesh.cells.size() * p->scheme->Stride();
+    for (int s = 0; s < n_steps; ++s)
+      Rk3Step(size, u, dt, [&](double const *x, double *r) { p->scheme->Residual(x, r); });
+  });
+}
+double oracle_dg_time_step(void *h, double const *u, double cfl, double dt_guess) {
+  return static_cast<DgHandle *>(h)->scheme->TimeStep(u, cfl, dt_guess);
+}
+void oracle_dg_approximate(void *h, double const *values, double *u) {
+  static_cast<DgHandle *>(h)->scheme->Approximate(values, u);
+}
+// Value of cell c's expansion at a global point.
+void oracle_dg_value(void *h, double const *u, int64_t c, double const *x, double *v) {
+  Val5 o = static_cast<DgHandle *>(h)->scheme->Value(c, u, Vec3{ x[0], x[1], x[2] });
+  for (int k = 0; k < 5; ++k) v[k] = o[k];
 }
 
 }  // extern "C"

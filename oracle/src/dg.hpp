@@ -1,0 +1,231 @@
+// Oracle (test infrastructure): modal DG on hexahedra and tetrahedra,
+// restated from spatial/dg/general.hpp:51-202 over spatial/fem.hpp:224-435,
+// polynomial/projection.hpp:28-164, basis/linear.hpp:87-141,
+// basis/taylor.hpp:198-208,394-429 and integrator/function.hpp:139-168.
+//
+// Each cell carries an orthonormal basis phi = S T(x - c): T the Taylor
+// monomials of degree <= p in global coordinates, c the cell centre and S
+// the lower-triangular Gram-Schmidt factor built from the cell's own
+// quadrature.  The state is the K x N coefficient block of every cell,
+// K-contiguous per basis function (expansion.hpp:83-104); no mass matrix is
+// needed because the basis is orthonormal.
+//
+// Pinned by (tests/test_oracle_dg.py): the simplex rules' monomial exactness
+// and the reference's digits; orthonormality of each cell's basis under its
+// quadrature and exact projection of polynomials of degree <= p; the
+// closed-form residual R_i = -(div F) sqrt(V) delta_i0 of a linear pressure
+// field with constant density and velocity (flux-of-own-state boundaries);
+// the first-order finite-volume update that piecewise-constant data must
+// give, recomputed face by face; free-stream preservation, conservation
+// with walls, and the sampled time step.
+#ifndef ORACLE_DG_HPP_
+#define ORACLE_DG_HPP_
+
+#include <cmath>
+#include <stdexcept>
+#include <vector>
+
+#include "core.hpp"
+#include "fr.hpp"
+#include "mesh.hpp"
+
+namespace oracle {
+
+// basis/taylor.hpp:198-208 (degree 1 and 2 in three dimensions).
+inline int TaylorSize(int p) { return p == 1 ? 4 : 10; }
+inline void TaylorValues(int p, Vec3 const &x, double *t) {
+  t[0] = 1; t[1] = x[0]; t[2] = x[1]; t[3] = x[2];
+  if (p >= 2) {
+    t[4] = x[0] * x[0]; t[5] = x[0] * x[1]; t[6] = x[0] * x[2];
+    t[7] = x[1] * x[1]; t[8] = x[1] * x[2]; t[9] = x[2] * x[2];
+  }
+}
+
+struct ModalBasis {  // basis::OrthoNormal
+  int p = 2, n = 10;
+  Vec3 center{};
+  double s[10][10] = {};  // lower triangular
+
+  // integrator/function.hpp:139-168 (OrthoNormalize) on the cell quadrature.
+  void Build(int degree, Vec3 const &c, std::vector<Vec3> const &qp, std::vector<double> const &qw) {
+    p = degree; n = TaylorSize(p); center = c;
+    double a[10][10] = {};
+    for (size_t q = 0; q < qp.size(); ++q) {
+      double t[10];
+      TaylorValues(p, sub(qp[q], center), t);
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) a[i][j] += t[i] * t[j] * qw[q];
+    }
+    for (int i = 0; i < n; ++i) for (int j = 0; j < n; ++j) s[i][j] = (i == j);
+    s[0][0] = 1 / std::sqrt(a[0][0]);
+    for (int i = 1; i < n; ++i) {
+      for (int j = 0; j < i; ++j) {
+        double temp = 0;
+        for (int k = 0; k <= j; ++k) temp += s[j][k] * a[k][i];
+        for (int l = 0; l <= j; ++l) s[i][l] -= temp * s[j][l];
+      }
+      double norm_sq = 0;
+      for (int j = 0; j <= i; ++j) {
+        double sum = 0, sij = s[i][j];
+        for (int k = 0; k < j; ++k) sum += 2 * s[i][k] * a[k][j];
+        norm_sq += sij * (sij * a[j][j] + sum);
+      }
+      double inv = std::sqrt(norm_sq);
+      for (int j = 0; j <= i; ++j) s[i][j] /= inv;
+    }
+  }
+  // OrthoNormal::operator(): S T(x - c).
+  void Values(Vec3 const &x, double *phi) const {
+    double t[10];
+    TaylorValues(p, sub(x, center), t);
+    for (int i = 0; i < n; ++i) {
+      double v = 0;
+      for (int j = 0; j <= i; ++j) v += s[i][j] * t[j];
+      phi[i] = v;
+    }
+  }
+  // Taylor::GetGradValue (taylor.hpp:394-429): grad[i][d] = d phi_i / d x_d.
+  void Grads(Vec3 const &xg, double (*g)[3]) const {
+    Vec3 x = sub(xg, center);
+    for (int i = 0; i < n; ++i) {
+      g[i][0] = s[i][1]; g[i][1] = s[i][2]; g[i][2] = s[i][3];
+      if (p >= 2) {
+        g[i][0] += s[i][4] * (2 * x[0]); g[i][0] += s[i][5] * x[1]; g[i][0] += s[i][6] * x[2];
+        g[i][1] += s[i][5] * x[0]; g[i][1] += s[i][7] * (2 * x[1]); g[i][1] += s[i][8] * x[2];
+        g[i][2] += s[i][6] * x[0]; g[i][2] += s[i][8] * x[1]; g[i][2] += s[i][9] * (2 * x[2]);
+      }
+    }
+  }
+};
+
+struct DgScheme {
+  Mesh const *mesh;
+  int p, n;
+  RotatedEuler riemann;
+  std::vector<BcEntry> bcs;
+  std::vector<ModalBasis> basis;
+
+  DgScheme(Mesh const *m, int degree, int riemann_kind, std::vector<BcEntry> bc)
+      : mesh(m), p(degree), n(TaylorSize(degree)), bcs(std::move(bc)) {
+    if (degree < 1 || degree > 2) throw std::runtime_error("modal DG: degree must be 1 or 2");
+    riemann.kind = RiemannKind(riemann_kind);
+    basis.resize(mesh->cells.size());
+    for (size_t c = 0; c < mesh->cells.size(); ++c) {
+      auto const &cell = mesh->cells[c];
+      basis[c].Build(p, cell.center, cell.qp, cell.qw);
+    }
+  }
+  size_t Stride() const { return size_t(5) * n; }
+
+  // Projection::GlobalToValue: u = C phi(x).
+  Val5 Value(int64_t c, double const *u, Vec3 const &x) const {
+    double phi[10];
+    basis[c].Values(x, phi);
+    Val5 v{ 0, 0, 0, 0, 0 };
+    double const *cf = u + c * Stride();
+    for (int k = 0; k < 5; ++k) {
+      double s = 0;
+      for (int i = 0; i < n; ++i) s += cf[5 * i + k] * phi[i];
+      v[k] = s;
+    }
+    return v;
+  }
+  // Polynomial::AddToResidual of flux * phi(x)^T.
+  void AddRankOne(int64_t c, Val5 const &flux, Vec3 const &x, double *r) const {
+    double phi[10];
+    basis[c].Values(x, phi);
+    double *rc = r + c * Stride();
+    for (int i = 0; i < n; ++i)
+      for (int k = 0; k < 5; ++k) rc[5 * i + k] += flux[k] * phi[i];
+  }
+
+  // fem.hpp:237-247 (GetResidualColumn) with dg/general.hpp:58-89, 92-202.
+  void Residual(double const *u, double *r) {
+    std::fill(r, r + mesh->cells.size() * Stride(), 0.0);
+    for (size_t c = 0; c < mesh->cells.size(); ++c) {  // AddFluxDivergence
+      auto const &cell = mesh->cells[c];
+      double *rc = r + c * Stride();
+      for (size_t q = 0; q < cell.qp.size(); ++q) {
+        Val5 v = Value(c, u, cell.qp[q]);
+        Flux53 f = GetFluxMatrix(v);
+        for (auto &row : f) for (auto &x : row) x *= cell.qw[q];
+        double g[10][3];
+        basis[c].Grads(cell.qp[q], g);
+        for (int i = 0; i < n; ++i)
+          for (int k = 0; k < 5; ++k) rc[5 * i + k] += f[k][0] * g[i][0] + f[k][1] * g[i][1] + f[k][2] * g[i][2];
+      }
+    }
+    for (int64_t fi = 0; fi < mesh->n_interior; ++fi) {  // AddFluxToHolderAndSharer
+      auto const &face = mesh->faces[fi];
+      for (size_t q = 0; q < face.qp.size(); ++q) {
+        Vec3 xs = add(face.qp[q], face.shift);
+        Val5 uh = Value(face.holder, u, face.qp[q]);
+        Val5 us = Value(face.sharer, u, xs);
+        Val5 flux = riemann.GetFluxUpwind(face.frames[q], uh, us);
+        for (auto &x : flux) x *= -face.qw[q];
+        AddRankOne(face.holder, flux, face.qp[q], r);
+        for (auto &x : flux) x = -x;
+        AddRankOne(face.sharer, flux, xs, r);
+      }
+    }
+    // fem.hpp:261-323: supersonic inlets, outlets, (subsonic: none), walls, smart.
+    const int family_order[] = { kBcSupersonicInlet, kBcSupersonicOutlet, kBcInviscidWall, kBcSmart };
+    for (int family : family_order) {
+      for (size_t fi = mesh->n_interior; fi < mesh->faces.size(); ++fi) {
+        auto const &face = mesh->faces[fi];
+        if (face.tag < 0 || face.tag >= (int)bcs.size()) throw std::runtime_error("boundary tag without BC");
+        if (bcs[face.tag].kind != family) continue;
+        for (size_t q = 0; q < face.qp.size(); ++q) {
+          Val5 flux;
+          auto const &frame = face.frames[q];
+          if (family == kBcSupersonicInlet) {
+            flux = riemann.GetFluxOfState(frame, bcs[face.tag].state);
+          } else {
+            Val5 uh = Value(face.holder, u, face.qp[q]);
+            if (family == kBcSupersonicOutlet) flux = riemann.GetFluxOfState(frame, uh);
+            else if (family == kBcInviscidWall) flux = riemann.GetFluxOnInviscidWall(frame, uh);
+            else flux = riemann.GetFluxOnSmartBoundary(frame, uh, bcs[face.tag].state);
+          }
+          for (auto &x : flux) x *= -face.qw[q];
+          AddRankOne(face.holder, flux, face.qp[q], r);
+        }
+      }
+    }
+  }
+
+  // projection.hpp:28-43 (Project): coefficients of f given at the cell
+  // quadrature points, values[cell][q][5].
+  void Approximate(double const *values, double *u) const {
+    size_t off = 0;
+    for (size_t c = 0; c < mesh->cells.size(); ++c) {
+      auto const &cell = mesh->cells[c];
+      double *cf = u + c * Stride();
+      std::fill(cf, cf + Stride(), 0.0);
+      for (size_t q = 0; q < cell.qp.size(); ++q, off += 5) {
+        double phi[10];
+        basis[c].Values(cell.qp[q], phi);
+        for (int i = 0; i < n; ++i)
+          for (int k = 0; k < 5; ++k) {
+            double prod = values[off + k] * phi[i];
+            prod *= cell.qw[q];
+            cf[5 * i + k] += prod;
+          }
+      }
+    }
+  }
+
+  // fem.hpp:417-435: the first N quadrature points of each cell are sampled.
+  double TimeStep(double const *u, double cfl, double dt_guess) const {
+    double min_dt = 1e100;
+    for (size_t c = 0; c < mesh->cells.size(); ++c) {
+      double max_speed = 1e-5;
+      for (int i = 0; i < n; ++i) max_speed = std::max(max_speed, GetMaximumSpeed(Value(c, u, mesh->cells[c].qp[i])));
+      min_dt = std::min(min_dt, mesh->cells[c].length / max_speed);
+    }
+    return std::min(cfl * min_dt, dt_guess);
+  }
+};
+
+}  // namespace oracle
+
+#endif  // ORACLE_DG_HPP_

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "core.hpp"
+#include "simplex.hpp"
 
 namespace oracle {
 
@@ -123,10 +124,64 @@ struct QuadMap {  // coordinate/quadrangle.hpp (Quadrangle4) + face.hpp:54-62
   }
 };
 
+// coordinate/tetrahedron.hpp:136-148: local nodes (1,0,0), (0,1,0), (0,0,1),
+// (0,0,0); faces listed so that the face map's normal points out of a
+// CGNS-ordered TETRA_4.
+constexpr int kTetFaces[4][3] = { { 0, 2, 1 }, { 0, 1, 3 }, { 0, 3, 2 }, { 1, 2, 3 } };
+
+// coordinate/tetrahedron.hpp:77-97 (SortNodesOnFace): the face id is the
+// sum of the cell-local indices of its three nodes minus 3.
+inline int TetGetFaceId(int64_t const *cell_nodes, int64_t const *face_nodes) {
+  int sum = 0, cnt = 0;
+  for (int c = 0; c < 4 && cnt < 3; ++c)
+    for (int f = 0; f < 3; ++f)
+      if (face_nodes[f] == cell_nodes[c]) { sum += c; ++cnt; break; }
+  if (cnt != 3) throw std::runtime_error("TetGetFaceId: not a face");
+  return sum - 3;
+}
+
+struct TetMap {  // coordinate/tetrahedron.hpp:56-130 (Tetrahedron4), cell.hpp:82-90
+  Vec3 x[4];
+  Vec3 LocalToGlobal(double a, double b, double c) const {
+    const double sh[4] = { a, b, c, 1.0 - a - b - c };
+    Vec3 s = scale(x[0], sh[0]);
+    for (int i = 1; i < 4; ++i) s = add(s, scale(x[i], sh[i]));
+    return s;
+  }
+  Mat3 Jacobian() const {  // rows d/dxi, d/deta, d/dzeta
+    Mat3 j;
+    for (int r = 0; r < 3; ++r) j[r] = sub(x[r], x[3]);
+    return j;
+  }
+};
+
+struct TriMap {  // coordinate/triangle.hpp (Triangle3) + face.hpp:54-62
+  Vec3 x[3];
+  Vec3 LocalToGlobal(double a, double b) const {
+    Vec3 s = scale(x[0], a);
+    s = add(s, scale(x[1], b));
+    return add(s, scale(x[2], 1.0 - a - b));
+  }
+  Frame NormalFrame() const {
+    Vec3 ra = sub(x[0], x[2]), rb = sub(x[1], x[2]);
+    Frame f;
+    f.a = normalized(cross(ra, rb));
+    f.b = normalized(ra);
+    f.c = cross(f.a, f.b);
+    return f;
+  }
+  double AreaFactor() const {
+    Vec3 ra = sub(x[0], x[2]), rb = sub(x[1], x[2]);
+    double g11 = dot(ra, ra), g12 = dot(ra, rb), g22 = dot(rb, rb);
+    return std::sqrt(g11 * g22 - g12 * g12);
+  }
+};
+
 struct CellGeom {
   int type;
   int64_t nodes[8];
   HexMap map;              // hexahedra
+  TetMap tmap;             // tetrahedra
   Vec3 center;
   double volume, length;   // length = cbrt(volume) / 2 (part.hpp:230)
   std::vector<Vec3> qp;    // quadrature points (global)
@@ -140,7 +195,8 @@ struct FaceGeom {
   int tag;                 // boundary tag (interior faces: -1)
   int64_t nodes[4];        // sorted w.r.t. the holder (SortNodesOnFace)
   Vec3 shift;              // sharer-side coordinates = holder-side + shift
-  QuadMap map;
+  QuadMap map;             // quadrangles (hexahedral meshes)
+  TriMap tmap;             // triangles (tetrahedral meshes)
   std::vector<Vec3> qp;
   std::vector<double> qw;
   std::vector<Frame> frames;
@@ -203,10 +259,50 @@ inline void BuildQuadFaceGeometry(Mesh *mesh, FaceGeom *face) {
   }
 }
 
+// integrator/tetrahedron.hpp:89-92 + element.hpp:102-115 with the 14-point
+// rule.  The weight is |det J| x local weight: the reference element's node
+// order is left-handed with respect to CGNS TETRA_4, so the signed
+// determinant is negative for a CGNS-ordered cell, while the reference's own
+// test (test/integrator/tetrahedron.cpp:25) expects the positive volume.
+inline void BuildTetCellGeometry(Mesh *mesh, CellGeom *cell) {
+  static const TetRule14 rule;
+  for (int i = 0; i < 4; ++i) cell->tmap.x[i] = mesh->coords[cell->nodes[i]];
+  cell->center = cell->tmap.LocalToGlobal(0.25, 0.25, 0.25);
+  cell->qp.clear(); cell->qw.clear(); cell->qw_local.clear();
+  const double det = std::abs(det3(cell->tmap.Jacobian()));
+  double vol = 0;
+  for (int q = 0; q < 14; ++q) {
+    cell->qp.push_back(cell->tmap.LocalToGlobal(rule.x[q][0], rule.x[q][1], rule.x[q][2]));
+    double wg = rule.w[q] * det;
+    cell->qw.push_back(wg);
+    cell->qw_local.push_back(rule.w[q]);
+    vol += wg;
+  }
+  cell->volume = vol;
+  cell->length = std::cbrt(vol) * 0.5;
+}
+
+inline void BuildTriFaceGeometry(Mesh *mesh, FaceGeom *face) {
+  static const TriRule6 rule;
+  for (int i = 0; i < 3; ++i) face->tmap.x[i] = mesh->coords[face->nodes[i]];
+  face->center = face->tmap.LocalToGlobal(1.0 / 3, 1.0 / 3);
+  const double area = face->tmap.AreaFactor();
+  const Frame frame = face->tmap.NormalFrame();
+  for (int q = 0; q < 6; ++q) {
+    face->qp.push_back(face->tmap.LocalToGlobal(rule.x[q][0], rule.x[q][1]));
+    face->qw.push_back(rule.w[q] * area);
+    face->frames.push_back(frame);
+  }
+}
+
 // Build the mesh: cells, interior faces (adjacency order), boundary faces.
-inline Mesh BuildHexMesh(MeshInput const &in, int q, bool lobatto) {
+// Hexahedra use q x q x q Gauss points (Lobatto if asked); tetrahedra the
+// 14-point rule with 6-point triangles on their faces.
+inline Mesh BuildMesh(MeshInput const &in, int q, bool lobatto) {
   Mesh mesh;
-  mesh.cell_type = kHexa8;
+  const bool tet = in.cell_type == kTetra4;
+  const int npe = tet ? 4 : 8, fnpe = tet ? 3 : 4;
+  mesh.cell_type = tet ? kTetra4 : kHexa8;
   mesh.q = q;
   mesh.lobatto = lobatto;
   if (lobatto) LobattoRule(q, mesh.x1d, mesh.w1d); else LegendreRule(q, mesh.x1d, mesh.w1d);
@@ -217,29 +313,31 @@ inline Mesh BuildHexMesh(MeshInput const &in, int q, bool lobatto) {
   std::unordered_map<int64_t, std::vector<int64_t>> node_to_cells;
   for (int64_t c = 0; c < in.n_cells; ++c) {
     auto &cell = mesh.cells[c];
-    cell.type = kHexa8;
-    for (int i = 0; i < 8; ++i) {
-      cell.nodes[i] = in.cells[8 * c + i];
+    cell.type = mesh.cell_type;
+    for (int i = 0; i < npe; ++i) {
+      cell.nodes[i] = in.cells[npe * c + i];
       node_to_cells[cell.nodes[i]].push_back(c);
     }
-    BuildHexCellGeometry(&mesh, &cell);
+    if (tet) BuildTetCellGeometry(&mesh, &cell); else BuildHexCellGeometry(&mesh, &cell);
   }
   // Adjacency by >= 3 common nodes (mesh/shuffler.hpp:526-527), one entry
   // per pair (i < j) with the holder's local face id.
   std::map<std::pair<int64_t, int64_t>, std::vector<std::tuple<int, int, Vec3>>> adj;
   for (int64_t c = 0; c < in.n_cells; ++c) {
     std::map<int64_t, int> cnt;
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < npe; ++i)
       for (auto other : node_to_cells[mesh.cells[c].nodes[i]])
         if (other > c) cnt[other]++;
     for (auto [other, n] : cnt) {
       if (n < 3) continue;
       std::vector<int64_t> common;
-      for (int i = 0; i < 8; ++i)
-        for (int k = 0; k < 8; ++k)
+      for (int i = 0; i < npe; ++i)
+        for (int k = 0; k < npe; ++k)
           if (mesh.cells[c].nodes[i] == mesh.cells[other].nodes[k]) common.push_back(mesh.cells[c].nodes[i]);
-      int fh = HexGetFaceId(mesh.cells[c].nodes, common.data(), (int)common.size());
-      int fs = HexGetFaceId(mesh.cells[other].nodes, common.data(), (int)common.size());
+      int fh = tet ? TetGetFaceId(mesh.cells[c].nodes, common.data())
+                   : HexGetFaceId(mesh.cells[c].nodes, common.data(), (int)common.size());
+      int fs = tet ? TetGetFaceId(mesh.cells[other].nodes, common.data())
+                   : HexGetFaceId(mesh.cells[other].nodes, common.data(), (int)common.size());
       adj[{ c, other }].emplace_back(fh, fs, Vec3{ 0, 0, 0 });
     }
   }
@@ -248,16 +346,18 @@ inline Mesh BuildHexMesh(MeshInput const &in, int q, bool lobatto) {
   std::vector<BFace> bfaces;
   for (int64_t b = 0; b < in.n_bfaces; ++b) {
     BFace bf;
-    for (int i = 0; i < 4; ++i) bf.nodes[i] = in.bface_nodes[4 * b + i];
+    for (int i = 0; i < fnpe; ++i) bf.nodes[i] = in.bface_nodes[fnpe * b + i];
     bf.tag = in.bface_tags[b];
+    if (tet && bf.tag < 0) throw std::runtime_error("periodic faces are supported on hexahedra only");
     std::map<int64_t, int> cnt;
-    for (int i = 0; i < 4; ++i) for (auto c : node_to_cells[bf.nodes[i]]) cnt[c]++;
+    for (int i = 0; i < fnpe; ++i) for (auto c : node_to_cells[bf.nodes[i]]) cnt[c]++;
     bf.holder = -1;
-    for (auto [c, n] : cnt) if (n == 4) { bf.holder = c; break; }
+    for (auto [c, n] : cnt) if (n == fnpe) { bf.holder = c; break; }
     if (bf.holder < 0) throw std::runtime_error("boundary face without holder");
-    bf.face = HexGetFaceId(mesh.cells[bf.holder].nodes, bf.nodes, 4);
+    bf.face = tet ? TetGetFaceId(mesh.cells[bf.holder].nodes, bf.nodes)
+                  : HexGetFaceId(mesh.cells[bf.holder].nodes, bf.nodes, 4);
     bf.center = { 0, 0, 0 };
-    for (int i = 0; i < 4; ++i) bf.center = add(bf.center, scale(mesh.coords[bf.nodes[i]], 0.25));
+    for (int i = 0; i < fnpe; ++i) bf.center = add(bf.center, scale(mesh.coords[bf.nodes[i]], 1.0 / fnpe));
     bfaces.push_back(bf);
   }
   // Periodic pairing: translate the center of each periodic face by +L_d.
@@ -293,8 +393,9 @@ inline Mesh BuildHexMesh(MeshInput const &in, int q, bool lobatto) {
       FaceGeom face;
       face.holder = pair.first; face.sharer = pair.second;
       face.holder_face = fh; face.sharer_face = fs; face.tag = -1; face.shift = s;
-      for (int i = 0; i < 4; ++i) face.nodes[i] = mesh.cells[pair.first].nodes[kHexFaces[fh][i]];
-      BuildQuadFaceGeometry(&mesh, &face);
+      for (int i = 0; i < fnpe; ++i)
+        face.nodes[i] = mesh.cells[pair.first].nodes[tet ? kTetFaces[fh][i] : kHexFaces[fh][i]];
+      if (tet) BuildTriFaceGeometry(&mesh, &face); else BuildQuadFaceGeometry(&mesh, &face);
       mesh.faces.push_back(std::move(face));
     }
   }
@@ -305,11 +406,18 @@ inline Mesh BuildHexMesh(MeshInput const &in, int q, bool lobatto) {
     face.holder = bfaces[b].holder; face.sharer = -1;
     face.holder_face = bfaces[b].face; face.sharer_face = -1; face.tag = bfaces[b].tag;
     face.shift = { 0, 0, 0 };
-    for (int i = 0; i < 4; ++i) face.nodes[i] = mesh.cells[face.holder].nodes[kHexFaces[face.holder_face][i]];
-    BuildQuadFaceGeometry(&mesh, &face);
+    for (int i = 0; i < fnpe; ++i)
+      face.nodes[i] = mesh.cells[face.holder].nodes[tet ? kTetFaces[face.holder_face][i]
+                                                        : kHexFaces[face.holder_face][i]];
+    if (tet) BuildTriFaceGeometry(&mesh, &face); else BuildQuadFaceGeometry(&mesh, &face);
     mesh.faces.push_back(std::move(face));
   }
   return mesh;
+}
+
+inline Mesh BuildHexMesh(MeshInput const &in, int q, bool lobatto) {
+  if (in.cell_type != kHexa8) throw std::runtime_error("nodal FR needs hexahedra");
+  return BuildMesh(in, q, lobatto);
 }
 
 }  // namespace oracle

@@ -65,3 +65,53 @@ def box_hex(nx: int, ny: int, nz: int, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
         "bface_tags": np.ascontiguousarray(np.concatenate(tags), dtype=np.int32),
         "period": period,
     }
+
+
+# Six tetrahedra per hexahedron around the diagonal from corner 0 to corner 6
+# (CGNS HEXA_8 corner numbering); translation-invariant, hence conforming.
+_KUHN = ((0, 1, 2, 6), (0, 2, 3, 6), (0, 3, 7, 6), (0, 7, 4, 6), (0, 4, 5, 6), (0, 5, 1, 6))
+
+
+def box_tet(nx: int, ny: int, nz: int, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), jitter: float = 0.0,
+            seed: int = 0) -> dict:
+    """Hexahedral box split into 6 tetrahedra per hexahedron (TETRA_4, CGNS order).
+
+    ``jitter`` moves every vertex off the boundary by U(-jitter*h, +jitter*h)
+    per coordinate (h the spacing in that direction), seeded.  Every tetrahedron
+    is right-handed in CGNS order: (x1-x0) x (x2-x0) . (x3-x0) > 0.  Boundary
+    triangles carry the box-side tags 0..5 (x-, x+, y-, y+, z-, z+).
+    """
+    hexa = box_hex(nx, ny, nz, lo, hi)
+    coords = hexa["coords"].copy()
+    lo = np.asarray(lo, dtype=np.float64)
+    hi = np.asarray(hi, dtype=np.float64)
+    if jitter > 0:
+        rng = np.random.default_rng(seed)
+        h = (hi - lo) / np.array([nx, ny, nz])
+        inner = np.all((coords > lo + 0.5 * h) & (coords < hi - 0.5 * h), axis=1)
+        coords[inner] += rng.uniform(-jitter, jitter, (int(inner.sum()), 3)) * h
+    cells = hexa["cells"][:, np.array(_KUHN)].reshape(-1, 4)
+    x = coords[cells]
+    vol = np.einsum("ij,ij->i", np.cross(x[:, 1] - x[:, 0], x[:, 2] - x[:, 0]), x[:, 3] - x[:, 0])
+    flip = vol < 0
+    cells[flip] = cells[flip][:, [0, 2, 1, 3]]
+    # boundary triangles: faces used by one tetrahedron only
+    faces = np.concatenate([cells[:, [0, 1, 2]], cells[:, [0, 1, 3]], cells[:, [0, 2, 3]], cells[:, [1, 2, 3]]])
+    key = np.sort(faces, axis=1)
+    _, first, counts = np.unique(key, axis=0, return_index=True, return_counts=True)
+    bfaces = faces[np.sort(first[counts == 1])]
+    fx = coords[bfaces]
+    tags = np.full(len(bfaces), -1, dtype=np.int32)
+    eps = 1e-12 * (1 + np.abs(hi - lo).max())
+    for d in range(3):
+        tags[np.all(np.abs(fx[:, :, d] - lo[d]) < eps, axis=1)] = 2 * d
+        tags[np.all(np.abs(fx[:, :, d] - hi[d]) < eps, axis=1)] = 2 * d + 1
+    assert np.all(tags >= 0)
+    return {
+        "cell_type": 1,
+        "coords": np.ascontiguousarray(coords),
+        "cells": np.ascontiguousarray(cells, dtype=np.int64),
+        "bface_nodes": np.ascontiguousarray(bfaces, dtype=np.int64),
+        "bface_tags": tags,
+        "period": np.zeros(3),
+    }

@@ -204,3 +204,100 @@ class FrOracle:
         u = f64(u)
         r = np.zeros_like(u)
         return load().oracle_fr_time_residuals(self.h, _ptr(u, _dp), _ptr(r, _dp), n)
+
+
+class DgOracle:
+    """Modal DG on hexahedra or tetrahedra (spatial/dg/general.hpp restated)."""
+
+    def __init__(self, mesh: dict, degree: int, riemann: int, bcs=()):
+        lib = load()
+        if not getattr(lib, "_dg_bound", False):
+            lib.oracle_dg_create.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, ctypes.c_int64, _i64p,
+                                             ctypes.c_int64, _i64p, _i32p, _dp, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, _i32p, _dp, ctypes.POINTER(ctypes.c_void_p)]
+            lib.oracle_dg_destroy.argtypes = [ctypes.c_void_p]
+            lib.oracle_dg_sizes.argtypes = [ctypes.c_void_p, _i64p]
+            lib.oracle_dg_cells.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]
+            lib.oracle_dg_faces.argtypes = [ctypes.c_void_p, _i64p, _i64p, _i32p, _i32p, _i32p, _dp, _dp, _dp]
+            lib.oracle_dg_residual.argtypes = [ctypes.c_void_p, _dp, _dp]
+            lib.oracle_dg_rk3.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, ctypes.c_int]
+            lib.oracle_dg_time_step.restype = ctypes.c_double
+            lib.oracle_dg_time_step.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, ctypes.c_double]
+            lib.oracle_dg_approximate.argtypes = [ctypes.c_void_p, _dp, _dp]
+            lib.oracle_dg_value.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int64, _dp, _dp]
+            lib._dg_bound = True
+        self.mesh = mesh
+        cells = np.ascontiguousarray(mesh["cells"], np.int64)
+        self._keep = [f64(mesh["coords"]), cells, np.ascontiguousarray(mesh["bface_nodes"], np.int64),
+                      np.ascontiguousarray(mesh["bface_tags"], np.int32), f64(mesh["period"])]
+        kinds = np.array([b[0] for b in bcs] or [0], dtype=np.int32)
+        states = f64(np.array([b[1] for b in bcs] or [[0.0] * 5]).reshape(-1))
+        self._keep += [kinds, states]
+        c, cl, bn, bt, per = self._keep[:5]
+        h = ctypes.c_void_p()
+        _check(lib.oracle_dg_create(int(mesh.get("cell_type", 0)), len(c), _ptr(c, _dp), len(cl), _ptr(cl, _i64p),
+                                    len(bn), _ptr(bn, _i64p), _ptr(bt, _i32p), _ptr(per, _dp), degree,
+                                    int(riemann), len(bcs), _ptr(kinds, _i32p), _ptr(states, _dp),
+                                    ctypes.byref(h)))
+        self.h = h
+        sizes = np.zeros(5, np.int64)
+        lib.oracle_dg_sizes(h, _ptr(sizes, _i64p))
+        self.n_cells, self.n, self.nq, self.n_faces, self.n_interior = (int(v) for v in sizes)
+        self.degree = degree
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().oracle_dg_destroy(self.h)
+            self.h = None
+
+    @property
+    def size(self):
+        return self.n_cells * self.n * 5
+
+    def cells(self):
+        xyz = np.zeros((self.n_cells, self.nq, 3))
+        w = np.zeros((self.n_cells, self.nq))
+        centers = np.zeros((self.n_cells, 3))
+        s = np.zeros((self.n_cells, self.n, self.n))
+        lengths = np.zeros(self.n_cells)
+        load().oracle_dg_cells(self.h, _ptr(xyz, _dp), _ptr(w, _dp), _ptr(centers, _dp), _ptr(s, _dp),
+                               _ptr(lengths, _dp))
+        return {"xyz": xyz, "w": w, "centers": centers, "s": s, "lengths": lengths}
+
+    def faces(self):
+        nf = self.n_faces
+        nfp = 6 if self.mesh.get("cell_type", 0) == 1 else (self.degree + 1) ** 2
+        holder, sharer = np.zeros(nf, np.int64), np.zeros(nf, np.int64)
+        hf, sf, tag = np.zeros(nf, np.int32), np.zeros(nf, np.int32), np.zeros(nf, np.int32)
+        xyz, w, frames = np.zeros((nf, nfp, 3)), np.zeros((nf, nfp)), np.zeros((nf, nfp, 3, 3))
+        load().oracle_dg_faces(self.h, _ptr(holder, _i64p), _ptr(sharer, _i64p), _ptr(hf, _i32p),
+                               _ptr(sf, _i32p), _ptr(tag, _i32p), _ptr(xyz, _dp), _ptr(w, _dp), _ptr(frames, _dp))
+        return {"n_interior": self.n_interior, "holder": holder, "sharer": sharer, "holder_face": hf,
+                "sharer_face": sf, "tag": tag, "xyz": xyz, "w": w, "frames": frames}
+
+    def residual(self, u):
+        u = f64(u)
+        r = np.zeros_like(u)
+        _check(load().oracle_dg_residual(self.h, _ptr(u, _dp), _ptr(r, _dp)))
+        return r
+
+    def rk3(self, u, dt, n_steps=1):
+        u = f64(u).copy()
+        _check(load().oracle_dg_rk3(self.h, _ptr(u, _dp), float(dt), int(n_steps)))
+        return u
+
+    def time_step(self, u, cfl, dt_guess=1e300):
+        u = f64(u)
+        return load().oracle_dg_time_step(self.h, _ptr(u, _dp), cfl, dt_guess)
+
+    def approximate(self, values):
+        """values[cell][q][5] at the cell quadrature points -> coefficients."""
+        values = f64(values)
+        u = np.zeros(self.size)
+        load().oracle_dg_approximate(self.h, _ptr(values, _dp), _ptr(u, _dp))
+        return u
+
+    def value(self, u, cell, x):
+        u, x, v = f64(u), f64(x), np.zeros(5)
+        load().oracle_dg_value(self.h, _ptr(u, _dp), int(cell), _ptr(x, _dp), _ptr(v, _dp))
+        return v
