@@ -157,7 +157,8 @@ int mcfd_approximate(mcfd_solver *h, const double *values);
  * construction (mesh/part.hpp:932-975,1456-1549) and FR flux-point caches
  * (spatial/fr/general.hpp:85-211): per face holder, sharer (-1 on the
  * boundary), local face ids, and per face point the FluxPointCache::ijk of
- * the holder and of the sharer. */
+ * the holder and of the sharer (FR_GENERAL handles only; DG_GENERAL handles
+ * leave fp_ijk untouched, and any output pointer may be null). */
 int mcfd_face_count(const mcfd_solver *h, int64_t *n_faces, int64_t *n_interior);
 int mcfd_faces(const mcfd_solver *h, int64_t *holder, int64_t *sharer,
                int32_t *holder_face, int32_t *sharer_face, int32_t *fp_ijk);

@@ -1,0 +1,54 @@
+// One mesh discretised by modal DG (spatial/dg/general.hpp) on affine
+// hexahedra or tetrahedra: host connectivity, per-cell orthonormal bases and
+// geometry, face records, reference-element tables and the device arrays the
+// kernels in dg_modal.hip read.
+#ifndef MINICFD_HOST_DG_GROUP_HPP_
+#define MINICFD_HOST_DG_GROUP_HPP_
+
+#include <array>
+#include <cstdint>
+#include <vector>
+
+#include "../kernels/dg_modal.hpp"
+#include "common.hpp"
+
+namespace mcfd {
+
+struct DgFace {
+  int64_t holder, sharer;  // sharer < 0: boundary
+  int holder_face, sharer_face;
+  int orient;              // holder-sorted face node i sits at the sharer's sorted position Perm(orient, i)
+  int tag;                 // boundary tag (interior: -1)
+};
+
+struct DgGroup {
+  int cell_type = 0, degree = 0, riemann = 0;
+  int n = 0, nq = 0, nf = 0, nfp = 0;  // basis functions, volume points, faces and points per face
+  int64_t n_cells = 0, n_interior = 0;
+  std::vector<DgFace> faces;           // interior faces first, then boundary faces
+  DgTables tables{};
+  std::vector<DgTables> h_tables;     // upload source of `tables`
+  std::vector<double> h_basis, h_geo, h_faces, h_bc_state;
+  std::vector<int32_t> h_nbr, h_finfo, h_bc_kind;
+  std::vector<double> qp, qw;          // global quadrature points [cell][q][3] and weights [cell][q]
+  DeviceArray<double> u0, u1, u2, res, ct, basis, geo, face_rec, bc_state;
+  DeviceArray<int32_t> nbr, finfo, bc_kind;
+  DeviceArray<DgTables> tables_d;
+
+  void Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &scheme,
+             const std::vector<int32_t> &bc_kinds, const std::vector<double> &bc_states);
+  void Upload(hipStream_t stream);
+
+  size_t ColumnSize() const { return size_t(n_cells) * n * 5; }
+  // Stage 0 residual, 1..3 SSP-RK3 stages (Taylor coefficients, then the stage kernel).
+  void Stage(int stage, double dt, hipStream_t stream);
+  // min over cells of length / max wave speed at the first N quadrature points.
+  void TimeStepBits(unsigned long long *dt_min_bits, hipStream_t stream);
+  // Coefficients of f given at the quadrature points (polynomial/projection.hpp:28-43).
+  void Approximate(const double *values, double *coeffs) const;
+  void Faces(int64_t *holder, int64_t *sharer, int32_t *holder_face, int32_t *sharer_face) const;
+};
+
+}  // namespace mcfd
+
+#endif  // MINICFD_HOST_DG_GROUP_HPP_

@@ -9,6 +9,7 @@
 
 #include "../../../include/minicfd.h"
 #include "common.hpp"
+#include "dg_group.hpp"
 #include "fr_group.hpp"
 
 namespace mcfd {
@@ -27,14 +28,23 @@ using mcfd::Guard;
 struct mcfd_solver {
   double t = 0;
   bool host_only = false;
+  bool dg = false;  // modal DG (DgGroup) instead of nodal FR (FrHexGroup)
   hipStream_t stream = nullptr;
   mcfd::FrHexGroup group;
+  mcfd::DgGroup modal;
   mcfd::DeviceArray<unsigned long long> scratch;  // time-step reduction
 
   void Stage(int stage, double dt) {
+    if (dg) {
+      modal.Stage(stage, dt, stream);
+      return;
+    }
     Check(mcfd::LaunchFrHexStage(group.q, group.riemann, group.Args(stage, dt), group.lt, stream),
           "FR stage launch");
   }
+  size_t ColumnSize() const { return dg ? modal.ColumnSize() : group.ColumnSize(); }
+  double *Solution() { return dg ? modal.u0.ptr : group.u0.ptr; }
+  double *Residual() { return dg ? modal.res.ptr : group.res.ptr; }
 };
 
 #define MCFD_REQUIRE_DEVICE(h)                                                        \
@@ -53,20 +63,31 @@ int mcfd_create(const mcfd_mesh_desc *mesh, const mcfd_scheme_desc *scheme, cons
   *out = nullptr;
   auto h = std::make_unique<mcfd_solver>();
   int rc = Guard([&] {
-    if (scheme->scheme != MCFD_SCHEME_FR_GENERAL || mesh->cell_type != MCFD_HEXA_8)
-      throw mcfd::Failure{ MCFD_ENOTSUP, "this handle implements FR_GENERAL on HEXA_8" };
+    if (scheme->scheme == MCFD_SCHEME_FR_GENERAL && mesh->cell_type != MCFD_HEXA_8)
+      throw mcfd::Failure{ MCFD_ENOTSUP, "FR_GENERAL is implemented on HEXA_8" };
+    if (scheme->scheme != MCFD_SCHEME_FR_GENERAL && scheme->scheme != MCFD_SCHEME_DG_GENERAL)
+      throw mcfd::Failure{ MCFD_ENOTSUP, "unknown scheme" };
     std::vector<int32_t> kinds(n_bcs);
     std::vector<double> states(5 * n_bcs);
     for (int b = 0; b < n_bcs; ++b) {
       kinds[b] = bcs[b].kind;
       for (int k = 0; k < 5; ++k) states[5 * b + k] = bcs[b].state[k];
     }
-    h->group.Build(*mesh, *scheme, kinds, states);
+    h->dg = scheme->scheme == MCFD_SCHEME_DG_GENERAL;
+    if (h->dg) {
+      h->modal.Build(*mesh, *scheme, kinds, states);
+    } else {
+      h->group.Build(*mesh, *scheme, kinds, states);
+    }
     h->host_only = (scheme->flags & MCFD_FLAG_HOST_ONLY) != 0;
     if (h->host_only) return;
-    h->group.BuildTileOrder(mcfd::FrHexStageTileCells(h->group.q));
     Check(hipStreamCreate(&h->stream), "hipStreamCreate");
-    h->group.Upload(h->stream);
+    if (h->dg) {
+      h->modal.Upload(h->stream);
+    } else {
+      h->group.BuildTileOrder(mcfd::FrHexStageTileCells(h->group.q));
+      h->group.Upload(h->stream);
+    }
     h->scratch.Allocate(1);
     Check(hipStreamSynchronize(h->stream), "create sync");
   });
@@ -85,9 +106,9 @@ int mcfd_destroy(mcfd_solver *h) {
 
 int mcfd_sizes(const mcfd_solver *h, int64_t *n_cells, int32_t *n_per_cell, int64_t *column_size) {
   if (!h) return Fail(MCFD_EINVAL, "null handle");
-  if (n_cells) *n_cells = h->group.n_cells;
-  if (n_per_cell) *n_per_cell = h->group.np;
-  if (column_size) *column_size = (int64_t)h->group.ColumnSize();
+  if (n_cells) *n_cells = h->dg ? h->modal.n_cells : h->group.n_cells;
+  if (n_per_cell) *n_per_cell = h->dg ? h->modal.n : h->group.np;
+  if (column_size) *column_size = (int64_t)h->ColumnSize();
   return MCFD_OK;
 }
 
@@ -101,7 +122,7 @@ int mcfd_set_solution_column(mcfd_solver *h, const double *column) {
   MCFD_REQUIRE_DEVICE(h);
   if (!column) return Fail(MCFD_EINVAL, "null column");
   return Guard([&] {
-    Check(hipMemcpyAsync(h->group.u0.ptr, column, h->group.ColumnSize() * sizeof(double),
+    Check(hipMemcpyAsync(h->Solution(), column, h->ColumnSize() * sizeof(double),
                          hipMemcpyHostToDevice, h->stream), "set solution");
     Check(hipStreamSynchronize(h->stream), "set solution sync");
   });
@@ -111,7 +132,7 @@ int mcfd_get_solution_column(mcfd_solver *h, double *column) {
   MCFD_REQUIRE_DEVICE(h);
   if (!column) return Fail(MCFD_EINVAL, "null column");
   return Guard([&] {
-    Check(hipMemcpyAsync(column, h->group.u0.ptr, h->group.ColumnSize() * sizeof(double),
+    Check(hipMemcpyAsync(column, h->Solution(), h->ColumnSize() * sizeof(double),
                          hipMemcpyDeviceToHost, h->stream), "get solution");
     Check(hipStreamSynchronize(h->stream), "get solution sync");
   });
@@ -122,7 +143,7 @@ int mcfd_get_residual_column(mcfd_solver *h, double *column) {
   if (!column) return Fail(MCFD_EINVAL, "null column");
   return Guard([&] {
     h->Stage(0, 0.0);
-    Check(hipMemcpyAsync(column, h->group.res.ptr, h->group.ColumnSize() * sizeof(double),
+    Check(hipMemcpyAsync(column, h->Residual(), h->ColumnSize() * sizeof(double),
                          hipMemcpyDeviceToHost, h->stream), "get residual");
     Check(hipStreamSynchronize(h->stream), "get residual sync");
   });
@@ -156,8 +177,12 @@ int mcfd_get_time_step(mcfd_solver *h, double cfl, double dt_guess, double *dt) 
     unsigned long long bits;
     std::memcpy(&bits, &init, sizeof(bits));
     Check(hipMemcpyAsync(h->scratch.ptr, &bits, sizeof(bits), hipMemcpyHostToDevice, h->stream), "dt init");
-    Check(mcfd::LaunchFrHexTimeStep(h->group.q, h->group.u0.ptr, h->group.geo_id.ptr, h->group.geo.ptr,
-                                    h->group.n_cells, h->scratch.ptr, h->stream), "time step launch");
+    if (h->dg) {
+      h->modal.TimeStepBits(h->scratch.ptr, h->stream);
+    } else {
+      Check(mcfd::LaunchFrHexTimeStep(h->group.q, h->group.u0.ptr, h->group.geo_id.ptr, h->group.geo.ptr,
+                                      h->group.n_cells, h->scratch.ptr, h->stream), "time step launch");
+    }
     Check(hipMemcpyAsync(&bits, h->scratch.ptr, sizeof(bits), hipMemcpyDeviceToHost, h->stream), "dt read");
     Check(hipStreamSynchronize(h->stream), "dt sync");
     std::memcpy(&min_dt, &bits, sizeof(min_dt));
@@ -167,27 +192,46 @@ int mcfd_get_time_step(mcfd_solver *h, double cfl, double dt_guess, double *dt) 
 
 int mcfd_sample_points(const mcfd_solver *h, int32_t *n_per_cell, double *xyz) {
   if (!h) return Fail(MCFD_EINVAL, "null handle");
+  if (h->dg) {  // the cell quadrature points
+    if (n_per_cell) *n_per_cell = h->modal.nq;
+    if (xyz) std::copy(h->modal.qp.begin(), h->modal.qp.end(), xyz);
+    return MCFD_OK;
+  }
   if (n_per_cell) *n_per_cell = h->group.np;
   if (xyz) h->group.SamplePoints(xyz);
   return MCFD_OK;
 }
 
 int mcfd_approximate(mcfd_solver *h, const double *values) {
-  // Nodal FR: the solution points are the sample points
-  // (polynomial/hexahedron.hpp:646-652).
-  return mcfd_set_solution_column(h, values);
+  MCFD_REQUIRE_DEVICE(h);
+  if (!values) return Fail(MCFD_EINVAL, "null values");
+  if (!h->dg) {
+    // Nodal FR: the solution points are the sample points
+    // (polynomial/hexahedron.hpp:646-652).
+    return mcfd_set_solution_column(h, values);
+  }
+  std::vector<double> coeffs;
+  int rc = Guard([&] {
+    coeffs.resize(h->ColumnSize());
+    h->modal.Approximate(values, coeffs.data());
+  });
+  return rc != MCFD_OK ? rc : mcfd_set_solution_column(h, coeffs.data());
 }
 
 int mcfd_face_count(const mcfd_solver *h, int64_t *n_faces, int64_t *n_interior) {
   if (!h) return Fail(MCFD_EINVAL, "null handle");
-  if (n_faces) *n_faces = (int64_t)h->group.conn.faces.size();
-  if (n_interior) *n_interior = h->group.conn.n_interior;
+  if (n_faces) *n_faces = (int64_t)(h->dg ? h->modal.faces.size() : h->group.conn.faces.size());
+  if (n_interior) *n_interior = h->dg ? h->modal.n_interior : h->group.conn.n_interior;
   return MCFD_OK;
 }
 
 int mcfd_faces(const mcfd_solver *h, int64_t *holder, int64_t *sharer, int32_t *holder_face,
                int32_t *sharer_face, int32_t *fp_ijk) {
   if (!h) return Fail(MCFD_EINVAL, "null handle");
+  if (h->dg) {
+    h->modal.Faces(holder, sharer, holder_face, sharer_face);
+    return MCFD_OK;
+  }
   h->group.Faces(holder, sharer, holder_face, sharer_face, fp_ijk);
   return MCFD_OK;
 }

@@ -1,0 +1,328 @@
+// Modal DG stage kernels for affine hexahedra and tetrahedra (gfx950).
+//
+// Decomposition of one stage pass: a workgroup of 256 threads owns G cells;
+// each cell has T = NQ + NF * NFP point tasks (its volume quadrature points
+// and the quadrature points of its faces), one thread per task.
+//   phase 1  volume task: state u = Ct T(y) at the point, flux matrix times
+//            the global weight -> LDS;  face task: own trace, neighbour trace
+//            (from the neighbour's Taylor coefficients and Jacobian),
+//            Riemann or boundary flux times -/+ the weight -> LDS.
+//   phase 2  one thread per (cell, component, Taylor monomial) sums the
+//            moments Rt over the cell's tasks;
+//   phase 3  one thread per (cell, basis function, component): R = Rt S^T,
+//            the SSP-RK3 combination (temporal/rk.hpp:31-49 operation order)
+//            and the store.
+// An interior face is evaluated by both of its cells from identical inputs
+// (the same table entries, Jacobians and coefficients through the same
+// fixed-order FMA chains), so the two sides see bitwise the same flux and
+// the scheme stays conservative without atomics.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "dg_modal.hpp"
+#include "euler.hpp"
+
+namespace mcfd {
+namespace {
+
+constexpr int kSlot = 18;  // doubles per task in LDS: 15 flux + 3 offset
+
+template <bool kHex, int P>
+struct DgShape {
+  static constexpr int kN = P == 1 ? 4 : 10;
+  static constexpr int kNQ = kHex ? (P + 1) * (P + 1) * (P + 1) : 14;
+  static constexpr int kNF = kHex ? 6 : 4;
+  static constexpr int kNFP = kHex ? (P + 1) * (P + 1) : 6;
+  static constexpr int kT = kNQ + kNF * kNFP;
+  static constexpr int kG = 256 / kT;
+  static constexpr int kPacked = kN * (kN + 1) / 2;
+};
+
+// y = J^T eta: offset from the cell centre of the point with local
+// coordinates xi = xi_c + eta (J rows are d/dxi, d/deta, d/dzeta).
+__device__ __forceinline__ void Offset(const double *j, const double *eta, double *y) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) y[d] = __fma_rn(eta[2], j[6 + d], __fma_rn(eta[1], j[3 + d], __dmul_rn(eta[0], j[d])));
+}
+
+// u = Ct T(y), Ct stored [N][5].
+template <int P>
+__device__ __forceinline__ State EvalTaylor(const double *ct, const double *y) {
+  constexpr int N = P == 1 ? 4 : 10;
+  double t[10];
+  t[0] = 1.0; t[1] = y[0]; t[2] = y[1]; t[3] = y[2];
+  if constexpr (P >= 2) {
+    t[4] = __dmul_rn(y[0], y[0]); t[5] = __dmul_rn(y[0], y[1]); t[6] = __dmul_rn(y[0], y[2]);
+    t[7] = __dmul_rn(y[1], y[1]); t[8] = __dmul_rn(y[1], y[2]); t[9] = __dmul_rn(y[2], y[2]);
+  }
+  State u;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    double acc = ct[k];
+#pragma unroll
+    for (int j = 1; j < N; ++j) acc = __fma_rn(ct[5 * j + k], t[j], acc);
+    u.v[k] = acc;
+  }
+  return u;
+}
+
+// Exponents (a, b, c) of Taylor monomial j (basis/taylor.hpp:204-208 order).
+__device__ __forceinline__ void Exponents(int j, int *a, int *b, int *c) {
+  *a = (j == 1 || j == 5 || j == 6) ? 1 : (j == 4 ? 2 : 0);
+  *b = (j == 2 || j == 5 || j == 8) ? 1 : (j == 7 ? 2 : 0);
+  *c = (j == 3 || j == 6 || j == 8) ? 1 : (j == 9 ? 2 : 0);
+}
+
+__device__ __forceinline__ double Power(double x, int e) {  // e in {-1, 0, 1, 2}; x^-1 is never used
+  return e == 0 ? 1.0 : (e == 1 ? x : (e == 2 ? x * x : 0.0));
+}
+
+template <bool kHex, int P, int kRiemann>
+__global__ __launch_bounds__(256) void DgModalStageKernel(DgModalArgs a) {
+  using S = DgShape<kHex, P>;
+  constexpr int N = S::kN, NQ = S::kNQ, NFP = S::kNFP, T = S::kT, G = S::kG;
+  __shared__ double s_ct[G * 5 * N];
+  __shared__ double s_task[G * T * kSlot];
+  __shared__ double s_rt[G * 5 * N];
+
+  const int tid = threadIdx.x;
+  const int64_t c0 = (int64_t)blockIdx.x * G;
+  if (c0 >= a.n_cells) return;
+  const int nc = (int)min<int64_t>(G, a.n_cells - c0);
+
+  for (int w = tid; w < nc * 5 * N; w += 256) s_ct[w] = a.ct[c0 * 5 * N + w];
+  __syncthreads();
+
+  // ---- phase 1: point tasks
+  const int g = tid / T, r = tid - (tid / T) * T;
+  if (g < nc) {
+    const int64_t cid = c0 + g;
+    const double *geo = a.geo + cid * kDgGeoStride;
+    const double *own_ct = s_ct + g * 5 * N;
+    double *slot = s_task + (g * T + r) * kSlot;
+    const DgTables *tb = a.tables;
+    if (r < NQ) {
+      double y[3];
+      Offset(geo, tb->vol_eta[r], y);
+      const State u = EvalTaylor<P>(own_ct, y);
+      double f[5][3];
+      FluxMatrix(u, f);
+      const double w = __dmul_rn(tb->vol_w[r], geo[9]);
+#pragma unroll
+      for (int k = 0; k < 5; ++k)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) slot[3 * k + d] = f[k][d] * w;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) slot[15 + d] = y[d];
+    } else {
+      const int lf = (r - NQ) / NFP, q = (r - NQ) - ((r - NQ) / NFP) * NFP;
+      const int32_t nb = a.nbr[cid * 6 + lf];
+      const int32_t info = a.finfo[cid * 6 + lf];
+      const bool holder = (info >> 7) & 1;
+      const int orient = (info >> 3) & 15;
+      const double *fc = a.faces + (int64_t)(info >> 8) * kDgFaceStride;
+      FaceFrame frame;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) { frame.n[d] = fc[d]; frame.t[d] = fc[3 + d]; }
+      frame.b[0] = frame.n[1] * frame.t[2] - frame.n[2] * frame.t[1];
+      frame.b[1] = frame.n[2] * frame.t[0] - frame.n[0] * frame.t[2];
+      frame.b[2] = frame.n[0] * frame.t[1] - frame.n[1] * frame.t[0];
+      const double w = __dmul_rn(tb->face_w[q], fc[6]);
+      double y[3];
+      Offset(geo, tb->face_eta[lf][holder ? 0 : orient][q], y);
+      const State own = EvalTaylor<P>(own_ct, y);
+      State flux;
+      double sgn = -1.0;
+      if (nb >= 0) {
+        const int nf = info & 7;
+        double yn[3];
+        Offset(a.geo + (int64_t)nb * kDgGeoStride, tb->face_eta[nf][holder ? orient : 0][q], yn);
+        const State other = EvalTaylor<P>(a.ct + (int64_t)nb * 5 * N, yn);
+        flux = UpwindFlux<kRiemann>(frame, holder ? own : other, holder ? other : own);
+        sgn = holder ? -1.0 : 1.0;
+      } else {
+        const int tag = -nb - 1;
+        const int kind = a.bc_kind[tag];
+        State given;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) given.v[k] = a.bc_state[tag * 5 + k];
+        if (kind == 1) {
+          flux = WallFlux(frame, own);
+        } else if (kind == 2) {
+          flux = StateFlux(frame, own);
+        } else if (kind == 3) {
+          flux = StateFlux(frame, given);
+        } else {
+          flux = UpwindFlux<kRiemann>(frame, own, given);
+        }
+      }
+      // dg/general.hpp:80-88: flux *= -w into the holder, -(flux * -w) into the sharer
+#pragma unroll
+      for (int k = 0; k < 5; ++k) slot[k] = flux.v[k] * (sgn * w);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) slot[15 + d] = y[d];
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 2: Taylor moments Rt[k][j]
+  for (int w = tid; w < nc * 5 * N; w += 256) {
+    const int gc = w / (5 * N), rem = w - gc * 5 * N, j = rem / 5, k = rem - (rem / 5) * 5;
+    int ea, eb, ec;
+    Exponents(j, &ea, &eb, &ec);
+    const double *base = s_task + gc * T * kSlot;
+    double acc = 0.0;
+    for (int q = 0; q < NQ; ++q) {
+      const double *sl = base + q * kSlot;
+      const double x = sl[15], yv = sl[16], z = sl[17];
+      const double px = Power(x, ea), py = Power(yv, eb), pz = Power(z, ec);
+      const double gx = ea ? ea * Power(x, ea - 1) * py * pz : 0.0;
+      const double gy = eb ? eb * px * Power(yv, eb - 1) * pz : 0.0;
+      const double gz = ec ? ec * px * py * Power(z, ec - 1) : 0.0;
+      acc = __fma_rn(sl[3 * k], gx, acc);
+      acc = __fma_rn(sl[3 * k + 1], gy, acc);
+      acc = __fma_rn(sl[3 * k + 2], gz, acc);
+    }
+    for (int q = NQ; q < T; ++q) {
+      const double *sl = base + q * kSlot;
+      acc = __fma_rn(sl[k], Power(sl[15], ea) * Power(sl[16], eb) * Power(sl[17], ec), acc);
+    }
+    s_rt[w] = acc;
+  }
+  __syncthreads();
+
+  // ---- phase 3: R = Rt S^T, RK stage, store
+  for (int w = tid; w < nc * 5 * N; w += 256) {
+    const int gc = w / (5 * N), rem = w - gc * 5 * N, i = rem / 5, k = rem - (rem / 5) * 5;
+    const int64_t cid = c0 + gc;
+    const double *srow = a.basis + cid * S::kPacked + i * (i + 1) / 2;
+    const double *rt = s_rt + gc * 5 * N;
+    double res = 0.0;
+    for (int j = 0; j <= i; ++j) res = __fma_rn(rt[5 * j + k], srow[j], res);
+    const int64_t idx = cid * 5 * N + rem;
+    double out;
+    if (a.stage == 0) {
+      out = res;
+    } else {
+      const double next = __dadd_rn(a.us[idx], __dmul_rn(res, a.dt));
+      if (a.stage == 1) {
+        out = next;
+      } else if (a.stage == 2) {
+        out = __dadd_rn(__dmul_rn(a.u0[idx], 3.0), next) / 4.0;
+      } else {
+        out = __dadd_rn(__dmul_rn(next, 2.0), a.u0[idx]) / 3.0;
+      }
+    }
+    a.out[idx] = out;
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void DgTaylorKernel(const double *us, const double *basis, int64_t n_cells,
+                                                      double *ct) {
+  constexpr int N = P == 1 ? 4 : 10, PACKED = N * (N + 1) / 2;
+  const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (w >= n_cells * 5 * N) return;
+  const int64_t c = w / (5 * N);
+  const int rem = (int)(w - c * 5 * N), j = rem / 5, k = rem - (rem / 5) * 5;
+  const double *cf = us + c * 5 * N;
+  const double *s = basis + c * PACKED;
+  double acc = 0.0;
+  for (int i = j; i < N; ++i) acc = __fma_rn(cf[5 * i + k], s[i * (i + 1) / 2 + j], acc);
+  ct[w] = acc;
+}
+
+template <bool kHex, int P>
+__global__ __launch_bounds__(256) void DgTimeStepKernel(const double *ct, const double *geo, const DgTables *tb,
+                                                        int64_t n_cells, unsigned long long *dt_min_bits) {
+  constexpr int N = P == 1 ? 4 : 10;
+  __shared__ double s_min[256];
+  double local = 1e300;
+  for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < n_cells; c += (int64_t)gridDim.x * 256) {
+    const double *gc = geo + c * kDgGeoStride;
+    double vmax = 1e-5;  // fem.hpp:421
+    for (int q = 0; q < N; ++q) {
+      double y[3];
+      Offset(gc, tb->vol_eta[q], y);
+      vmax = fmax(vmax, MaxSpeed(EvalTaylor<P>(ct + c * 5 * N, y)));
+    }
+    local = fmin(local, gc[10] / vmax);
+  }
+  s_min[threadIdx.x] = local;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) s_min[threadIdx.x] = fmin(s_min[threadIdx.x], s_min[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMin(dt_min_bits, (unsigned long long)__double_as_longlong(s_min[0]));
+}
+
+template <bool kHex, int P, int R>
+hipError_t LaunchStage(const DgModalArgs &a, hipStream_t stream) {
+  constexpr int G = DgShape<kHex, P>::kG;
+  const int64_t blocks = (a.n_cells + G - 1) / G;
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((DgModalStageKernel<kHex, P, R>), dim3((unsigned)blocks), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+template <bool kHex, int P>
+hipError_t LaunchStageR(int riemann, const DgModalArgs &a, hipStream_t stream) {
+  switch (riemann) {
+  case kExact: return LaunchStage<kHex, P, kExact>(a, stream);
+  case kHllc: return LaunchStage<kHex, P, kHllc>(a, stream);
+  case kRoe: return LaunchStage<kHex, P, kRoe>(a, stream);
+  default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace
+
+hipError_t LaunchDgTaylorCoefficients(int degree, const double *us, const double *basis, int64_t n_cells,
+                                      double *ct, hipStream_t stream) {
+  const int n = degree == 1 ? 4 : 10;
+  const int64_t blocks = (n_cells * 5 * n + 255) / 256;
+  if (blocks < 1) return hipSuccess;
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  if (degree == 1) {
+    hipLaunchKernelGGL(DgTaylorKernel<1>, dim3((unsigned)blocks), dim3(256), 0, stream, us, basis, n_cells, ct);
+  } else if (degree == 2) {
+    hipLaunchKernelGGL(DgTaylorKernel<2>, dim3((unsigned)blocks), dim3(256), 0, stream, us, basis, n_cells, ct);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t LaunchDgModalStage(int cell_type, int degree, int riemann, const DgModalArgs &a, hipStream_t stream) {
+  if (a.n_cells < 1) return hipSuccess;
+  const bool hex = cell_type == 0;
+  if (hex && degree == 1) return LaunchStageR<true, 1>(riemann, a, stream);
+  if (hex && degree == 2) return LaunchStageR<true, 2>(riemann, a, stream);
+  if (!hex && degree == 1) return LaunchStageR<false, 1>(riemann, a, stream);
+  if (!hex && degree == 2) return LaunchStageR<false, 2>(riemann, a, stream);
+  return hipErrorInvalidValue;
+}
+
+hipError_t LaunchDgTimeStep(int cell_type, int degree, const double *ct, const double *geo, const DgTables *tables,
+                            int64_t n_cells, unsigned long long *dt_min_bits, hipStream_t stream) {
+  const int64_t blocks = std::min<int64_t>((n_cells + 255) / 256, 4096);
+  if (blocks < 1) return hipSuccess;
+  const dim3 grid((unsigned)blocks), block(256);
+  const bool hex = cell_type == 0;
+  if (hex && degree == 1) {
+    hipLaunchKernelGGL((DgTimeStepKernel<true, 1>), grid, block, 0, stream, ct, geo, tables, n_cells, dt_min_bits);
+  } else if (hex && degree == 2) {
+    hipLaunchKernelGGL((DgTimeStepKernel<true, 2>), grid, block, 0, stream, ct, geo, tables, n_cells, dt_min_bits);
+  } else if (!hex && degree == 1) {
+    hipLaunchKernelGGL((DgTimeStepKernel<false, 1>), grid, block, 0, stream, ct, geo, tables, n_cells, dt_min_bits);
+  } else if (!hex && degree == 2) {
+    hipLaunchKernelGGL((DgTimeStepKernel<false, 2>), grid, block, 0, stream, ct, geo, tables, n_cells, dt_min_bits);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace mcfd

@@ -1,0 +1,70 @@
+// Modal DG on affine hexahedra and tetrahedra (spatial/dg/general.hpp:51-202):
+// device tables, kernel arguments and launchers.
+//
+// Every cell carries the orthonormal basis phi = S T(x - c) of the reference
+// (basis/linear.hpp:87-141): T the Taylor monomials of degree p <= 2, S the
+// lower-triangular Gram-Schmidt factor.  The kernels work in the Taylor
+// space: a prep pass forms the Taylor coefficients Ct = C S of every cell
+// (so u(x) = Ct T(x - c)), the stage pass evaluates states at the volume and
+// face points from Ct, accumulates the residual's Taylor moments
+//   Rt[k][j] = sum_q w_q F(u_q)[k] . grad T_j(y_q) - sum_f w_f Fhat_f[k] T_j(y_f)
+// and maps them back with R = Rt S^T.  Points are kept in the cell's local
+// coordinates; y = J (xi - xi_c) is the offset from the cell centre.
+#ifndef MINICFD_KERNELS_DG_MODAL_HPP_
+#define MINICFD_KERNELS_DG_MODAL_HPP_
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mcfd {
+
+constexpr int kDgMaxVolumePoints = 27;  // 3^3 Gauss points (hexahedra, p = 2)
+constexpr int kDgMaxFacePoints = 9;     // 3^2 Gauss points (quadrangles, p = 2)
+constexpr int kDgMaxOrient = 8;         // dihedral group of the square
+constexpr int kDgGeoStride = 12;        // J rows (9), |det J| (1), length (1), pad
+constexpr int kDgFaceStride = 8;        // n (3), t (3), area factor (1), pad
+
+// Reference-element tables of one (cell type, degree); lives in device memory.
+struct DgTables {
+  double vol_eta[kDgMaxVolumePoints][3];  // xi_q - xi_c
+  double vol_w[kDgMaxVolumePoints];       // local weights
+  // face point q of a face whose holder-sorted node i sits at the cell's
+  // sorted face position Perm(orient, i): its local coordinates minus xi_c
+  double face_eta[6][kDgMaxOrient][kDgMaxFacePoints][3];
+  double face_w[kDgMaxFacePoints];        // local weights on the face
+};
+
+struct DgModalArgs {
+  const double *us;       // state of this stage [cell][N][5]
+  const double *u0;       // U0 for RK stages 2 and 3
+  double *out;            // residual (stage 0) or next state
+  const double *ct;       // Taylor coefficients of `us`, [cell][N][5]
+  const double *basis;    // [cell][N(N+1)/2] S, packed lower triangle by rows
+  const double *geo;      // [cell][kDgGeoStride]
+  const int32_t *nbr;     // [cell][6]: neighbour cell, or -(1 + boundary tag)
+  const int32_t *finfo;   // [cell][6]: face id << 8 | holder << 7 | orient << 3 | neighbour's local face
+  const double *faces;    // [face][kDgFaceStride]
+  const int32_t *bc_kind; // [tag]: 0 smart, 1 inviscid wall, 2 supersonic outlet, 3 inlet
+  const double *bc_state; // [tag][5]
+  const DgTables *tables;
+  int64_t n_cells;
+  double dt;
+  int32_t stage;          // 0 residual, 1..3 SSP-RK3 stage
+};
+
+// Ct = C S for every cell of `us` into `ct`.
+hipError_t LaunchDgTaylorCoefficients(int degree, const double *us, const double *basis, int64_t n_cells,
+                                      double *ct, hipStream_t stream);
+// One residual / RK-stage pass (reads a.ct, which must hold the Taylor
+// coefficients of a.us).
+hipError_t LaunchDgModalStage(int cell_type, int degree, int riemann, const DgModalArgs &a, hipStream_t stream);
+// min over cells of length / max_{q < N} (|u| + c)(x_q), as ordered bits of a
+// positive double folded into *dt_min_bits with atomicMin.
+hipError_t LaunchDgTimeStep(int cell_type, int degree, const double *ct, const double *geo,
+                            const DgTables *tables, int64_t n_cells, unsigned long long *dt_min_bits,
+                            hipStream_t stream);
+
+}  // namespace mcfd
+
+#endif  // MINICFD_KERNELS_DG_MODAL_HPP_

@@ -79,6 +79,7 @@ class FiniteElement(System):
         C.check(lib.mcfd_sizes(h, ctypes.byref(n_cells), ctypes.byref(n_per_cell), ctypes.byref(size)))
         self.n_cells, self.n_per_cell, self.column_size = n_cells.value, n_per_cell.value, size.value
         self.degree = degree
+        self.scheme = scheme
 
     def close(self):
         if getattr(self, "_h", None):
@@ -139,6 +140,8 @@ class FiniteElement(System):
         return out
 
     def approximate(self, values):
+        """Set the state from values at sample_points(): nodal values (FR) or
+        the L2 projection onto each cell's basis (DG, projection.hpp:28-43)."""
         values = np.ascontiguousarray(values, np.float64)
         C.check(C.load().mcfd_approximate(self._h, C.ptr(values, C._dp)))
 
@@ -146,14 +149,17 @@ class FiniteElement(System):
         nf, ni = ctypes.c_int64(), ctypes.c_int64()
         C.check(C.load().mcfd_face_count(self._h, ctypes.byref(nf), ctypes.byref(ni)))
         nf = nf.value
-        nfp = (self.degree + 1) ** 2
         holder, sharer = np.empty(nf, np.int64), np.empty(nf, np.int64)
         hf, sf = np.empty(nf, np.int32), np.empty(nf, np.int32)
-        ijk = np.empty((nf, nfp, 2), np.int32)
+        ijk = np.empty((nf, (self.degree + 1) ** 2, 2), np.int32) if self.scheme == "fr" else None
         C.check(C.load().mcfd_faces(self._h, C.ptr(holder, C._i64p), C.ptr(sharer, C._i64p),
-                                    C.ptr(hf, C._i32p), C.ptr(sf, C._i32p), C.ptr(ijk, C._i32p)))
-        return {"n_interior": ni.value, "holder": holder, "sharer": sharer, "holder_face": hf,
-                "sharer_face": sf, "fp_ijk": ijk}
+                                    C.ptr(hf, C._i32p), C.ptr(sf, C._i32p),
+                                    C.ptr(ijk, C._i32p) if ijk is not None else None))
+        out = {"n_interior": ni.value, "holder": holder, "sharer": sharer, "holder_face": hf,
+               "sharer_face": sf}
+        if ijk is not None:
+            out["fp_ijk"] = ijk
+        return out
 
 
 class Batch:

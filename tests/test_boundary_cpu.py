@@ -203,3 +203,29 @@ def test_meshgen_box_counts():
     o = ol.FrOracle(m, q=3, riemann=0, bcs=ORACLE_WALLS)
     f = o.faces()
     assert f["n_interior"] == 1584 and len(f["holder"]) - f["n_interior"] == 672
+
+
+# ---------------------------------------------------------------- modal DG handles
+DG_MESHES = {
+    "tets_jittered": (lambda: meshgen.box_tet(4, 3, 2, hi=(1.0, 0.6, 0.4), jitter=0.15, seed=7), True),
+    "hex_shuffled_walls": (lambda: meshgen.box_hex(5, 3, 2, hi=(1.0, 0.5, 0.3), shuffle_seed=6), True),
+    "hex_rotated_periodic": (lambda: rotate_cells(meshgen.box_hex(3, 3, 2, periodic=(True,) * 3), 14), False),
+}
+
+
+@pytest.mark.parametrize("name", sorted(DG_MESHES))
+@pytest.mark.parametrize("degree", [1, 2])
+def test_dg_connectivity_and_points_match_oracle(name, degree):
+    make, walls = DG_MESHES[name]
+    mesh = make()
+    bcs = WALLS if walls else ()
+    fe = spatial.FiniteElement(mesh, degree=degree, scheme="dg", riemann="hllc", bcs=bcs, host_only=True)
+    o = ol.DgOracle(mesh, degree=degree, riemann=1, bcs=ORACLE_WALLS if walls else ())
+    got, want = fe.faces(), o.faces()
+    assert got["n_interior"] == want["n_interior"]
+    for key in ("holder", "sharer", "holder_face", "sharer_face"):
+        np.testing.assert_array_equal(got[key], want[key], err_msg=key)
+    assert fe.n_per_cell == o.n and fe.column_size == o.size
+    c = o.cells()
+    np.testing.assert_allclose(fe.sample_points(), c["xyz"], rtol=0, atol=1e-14)
+    fe.close()
