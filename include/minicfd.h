@@ -69,6 +69,18 @@ extern "C" {
  * Used to inspect the face construction on machines without a GPU. */
 #define MCFD_FLAG_HOST_ONLY 1
 
+/* ---- limiters, applied whenever a solution column is set: after
+ * SetSolutionColumn, Approximate and every fused RK stage
+ * (spatial/with_limiter.hpp:56-66, limiter/reconstruct.hpp:22-47).
+ * WENO_FLAGGED: weno::Lazy reconstruction (limiter/weno.hpp:97-195) of the
+ *   cells that the troubled-cell indicator weno::Eigen::IsNotSmooth
+ *   (weno.hpp:230-258) flags.
+ * WENO_ALL: weno::Lazy on every cell (Lazy::IsNotSmooth is always true; the
+ *   configuration of demo/euler/sourceless.hpp:80-81). */
+#define MCFD_LIMITER_NONE 0
+#define MCFD_LIMITER_WENO_FLAGGED 1
+#define MCFD_LIMITER_WENO_ALL 2
+
 /* ---- boundary conditions, spatial/fem.hpp:196-222 (constant states) */
 #define MCFD_BC_SMART 0               /* SetSmartBoundary(name, f) */
 #define MCFD_BC_INVISCID_WALL 1       /* SetInviscidWall(name) */
@@ -101,6 +113,10 @@ typedef struct {
   int32_t precision;    /* MCFD_PRECISION_* */
   int32_t flags;        /* MCFD_FLAG_* */
   double c_next;        /* Vincent c_next: 1 = DG, p/(2p+1) = Huynh g2 */
+  int32_t limiter;      /* MCFD_LIMITER_* (DG_GENERAL only) */
+  int32_t reserved;
+  double weno_w0;       /* linear weight of each neighbour (sourceless.cpp:114: 0.001) */
+  double weno_eps;      /* smoothness offset (sourceless.cpp:114: 1e-6) */
 } mcfd_scheme_desc;
 
 typedef struct {
@@ -162,6 +178,11 @@ int mcfd_approximate(mcfd_solver *h, const double *values);
 int mcfd_face_count(const mcfd_solver *h, int64_t *n_faces, int64_t *n_interior);
 int mcfd_faces(const mcfd_solver *h, int64_t *holder, int64_t *sharer,
                int32_t *holder_face, int32_t *sharer_face, int32_t *fp_ijk);
+
+/* Troubled-cell flags of the last limiter application (1 = reconstructed),
+ * one byte per cell, and their count; limiter/reconstruct.hpp:29-34.  Either
+ * output may be null. */
+int mcfd_limiter_flags(mcfd_solver *h, uint8_t *flags, int64_t *n_troubled);
 
 /* Stream synchronisation and per-kernel timing of n RK3 steps with HIP
  * events on the handle's stream: total ms, summed ms of the stage kernels,

@@ -56,8 +56,10 @@ std::vector<BcEntry> MakeBcs(int n_bc, int32_t const *kinds, double const *state
 }
 
 // temporal/rk.hpp:31-49 with temporal/ode.hpp:121-128, same operation order.
-template <class Residual>
-void Rk3Step(size_t size, double *u, double dt, Residual &&residual) {
+// `set_solution` is the System's SetSolutionColumn side effect on a freshly
+// set column (a limiter, spatial/with_limiter.hpp:56-59), or nothing.
+template <class Residual, class SetSolution>
+void Rk3Step(size_t size, double *u, double dt, Residual &&residual, SetSolution &&set_solution) {
   std::vector<double> u_curr(u, u + size), r(size), u_next(size);
   auto next_solution = [&](double const *state, double *out) {  // Euler::NextSolution
     residual(state, r.data());
@@ -65,11 +67,18 @@ void Rk3Step(size_t size, double *u, double dt, Residual &&residual) {
   };
   std::vector<double> u1(size), t(size);
   next_solution(u, u1.data());
+  set_solution(u1.data());
   for (size_t i = 0; i < size; ++i) { u_next[i] = u_curr[i]; u_next[i] *= 3; }
   next_solution(u1.data(), t.data());
   for (size_t i = 0; i < size; ++i) { u_next[i] += t[i]; u_next[i] /= 4; }
+  set_solution(u_next.data());
   next_solution(u_next.data(), t.data());
   for (size_t i = 0; i < size; ++i) { t[i] *= 2; t[i] += u_curr[i]; t[i] /= 3; u[i] = t[i]; }
+  set_solution(u);
+}
+template <class Residual>
+void Rk3Step(size_t size, double *u, double dt, Residual &&residual) {
+  Rk3Step(size, u, dt, residual, [](double *) {});
 }
 
 }  // namespace
@@ -297,8 +306,32 @@ int oracle_dg_rk3(void *h, double *u, double dt, int n_steps) {
     auto *p = static_cast<DgHandle *>(h);
     size_t size = p->mesh.cells.size() * p->scheme->Stride();
     for (int s = 0; s < n_steps; ++s)
-      Rk3Step(size, u, dt, [&](double const *x, double *r) { p->scheme->Residual(x, r); });
+      Rk3Step(size, u, dt, [&](double const *x, double *r) { p->scheme->Residual(x, r); },
+              [&](double *x) { p->scheme->Limit(x); });
   });
+}
+// Limiter of the DG handle (see DgScheme::limiter for the modes).
+void oracle_dg_set_limiter(void *h, int mode, double w0, double eps) {
+  auto *p = static_cast<DgHandle *>(h);
+  p->scheme->limiter = mode; p->scheme->weno_w0 = w0; p->scheme->weno_eps = eps;
+}
+// Applies the limiter to u in place (as SetSolutionColumn would); flags[cell] gets 1 for
+// a reconstructed cell.  Returns the number of troubled cells.
+int64_t oracle_dg_limit(void *h, double *u, uint8_t *flags) {
+  auto *p = static_cast<DgHandle *>(h);
+  int64_t n = p->scheme->Limit(u);
+  std::copy(p->scheme->flags.begin(), p->scheme->flags.end(), flags);
+  return n;
+}
+// Flags of the last limiter application (inside rk3 too).
+void oracle_dg_last_flags(void *h, uint8_t *flags) {
+  auto *p = static_cast<DgHandle *>(h);
+  std::copy(p->scheme->flags.begin(), p->scheme->flags.end(), flags);
+}
+// Smoothness indicator (weno.hpp:74-95) of coefficients on cell c's basis.
+void oracle_dg_smoothness(void *h, int64_t c, double const *coeff, double *beta) {
+  Val5 b = static_cast<DgHandle *>(h)->scheme->Smoothness(c, coeff);
+  for (int k = 0; k < 5; ++k) beta[k] = b[k];
 }
 double oracle_dg_time_step(void *h, double const *u, double cfl, double dt_guess) {
   return static_cast<DgHandle *>(h)->scheme->TimeStep(u, cfl, dt_guess);

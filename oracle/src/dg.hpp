@@ -21,7 +21,9 @@
 #ifndef ORACLE_DG_HPP_
 #define ORACLE_DG_HPP_
 
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <stdexcept>
 #include <vector>
 
@@ -104,6 +106,14 @@ struct DgScheme {
   RotatedEuler riemann;
   std::vector<BcEntry> bcs;
   std::vector<ModalBasis> basis;
+  // Cell::adj_cells_ (mesh/part.hpp:962-963): neighbours in interior-face order.
+  std::vector<std::vector<int64_t>> adj;
+  // Limiter applied at every SetSolutionColumn (spatial/with_limiter.hpp:56-59):
+  // 0 none; 1 weno::Lazy reconstruction of the cells weno::Eigen::IsNotSmooth
+  // flags; 2 weno::Lazy on every cell (Lazy::IsNotSmooth is always true).
+  int limiter = 0;
+  double weno_w0 = 0.001, weno_eps = 1e-6;
+  std::vector<uint8_t> flags;  // of the last limiter application
 
   DgScheme(Mesh const *m, int degree, int riemann_kind, std::vector<BcEntry> bc)
       : mesh(m), p(degree), n(TaylorSize(degree)), bcs(std::move(bc)) {
@@ -113,6 +123,11 @@ struct DgScheme {
     for (size_t c = 0; c < mesh->cells.size(); ++c) {
       auto const &cell = mesh->cells[c];
       basis[c].Build(p, cell.center, cell.qp, cell.qw);
+    }
+    adj.resize(mesh->cells.size());
+    for (int64_t f = 0; f < mesh->n_interior; ++f) {
+      adj[mesh->faces[f].holder].push_back(mesh->faces[f].sharer);
+      adj[mesh->faces[f].sharer].push_back(mesh->faces[f].holder);
     }
   }
   size_t Stride() const { return size_t(5) * n; }
@@ -212,6 +227,153 @@ struct DgScheme {
           }
       }
     }
+  }
+
+  // ---------------------------------------------------------------- WENO limiter
+  // Projection::average (projection.hpp:45-50): C[:, 0] * S(0, 0).
+  Val5 Average(int64_t c, double const *coeff) const {
+    Val5 v;
+    for (int k = 0; k < 5; ++k) v[k] = coeff[k] * basis[c].s[0][0];
+    return v;
+  }
+  // limiter/weno.hpp:230-258 (weno::Eigen::IsNotSmooth) on rho and E.
+  bool IsNotSmooth(int64_t c, double const *u) const {
+    const int comps[2] = { 0, 4 };
+    Val5 max_abs = Average(c, u + c * Stride());
+    for (int i : comps) max_abs[i] = std::max(1e-9, std::abs(max_abs[i]));
+    double sum_abs[5] = { 0, 0, 0, 0, 0 };
+    const Vec3 &x = mesh->cells[c].center;
+    Val5 mine = Value(c, u, x);
+    for (int64_t a : adj[c]) {
+      Val5 theirs = Value(a, u, x);
+      Val5 avg = Average(a, u + a * Stride());
+      for (int i : comps) {
+        sum_abs[i] += std::abs(mine[i] - theirs[i]);
+        max_abs[i] = std::max(max_abs[i], std::abs(avg[i]));
+      }
+    }
+    const double volume_power = (p + 1.0) / 2.0 / 3;
+    double divisor = std::pow(mesh->cells[c].volume, volume_power);
+    divisor *= adj[c].size();
+    const double reference = p < 3 ? 1.0 : 3.0;
+    for (int i : comps)
+      if (sum_abs[i] / max_abs[i] / divisor > reference) return true;
+    return false;
+  }
+  // weno.hpp:74-95 (GetSmoothness) of coefficients on cell c's basis.
+  Val5 Smoothness(int64_t c, double const *coeff) const {
+    auto const &b = basis[c];
+    double tc[5][10] = {};  // Taylor coefficients C S
+    for (int k = 0; k < 5; ++k)
+      for (int j = 0; j < n; ++j) {
+        double v = 0;
+        for (int i = 0; i < n; ++i) v += coeff[5 * i + k] * b.s[i][j];
+        tc[k][j] = v;
+      }
+    auto const &cell = mesh->cells[c];
+    double integral[5][10] = {};
+    for (size_t q = 0; q < cell.qp.size(); ++q) {
+      Vec3 y = sub(cell.qp[q], b.center);
+      for (int k = 0; k < 5; ++k) {
+        double pdv[10];  // taylor.hpp:241-281 (GetPartialDerivatives)
+        pdv[0] = 0;
+        for (int j = 1; j < n; ++j) pdv[j] = tc[k][j];
+        if (p >= 2) {
+          pdv[1] += tc[k][4] * (2 * y[0]); pdv[1] += tc[k][5] * y[1]; pdv[1] += tc[k][6] * y[2];
+          pdv[2] += tc[k][5] * y[0]; pdv[2] += tc[k][7] * (2 * y[1]); pdv[2] += tc[k][8] * y[2];
+          pdv[3] += tc[k][6] * y[0]; pdv[3] += tc[k][8] * y[1]; pdv[3] += tc[k][9] * (2 * y[2]);
+          pdv[4] += tc[k][4]; pdv[7] += tc[k][7]; pdv[9] += tc[k][9];
+        }
+        for (int j = 0; j < n; ++j) integral[k][j] += pdv[j] * pdv[j] * cell.qw[q];
+      }
+    }
+    const double volume = 1 / (b.s[0][0] * b.s[0][0]);  // OrthoNormal::Measure
+    auto weight = [volume](int k) {
+      double f = std::tgamma(k + 1);
+      return std::pow(volume, k * 2. / 3 - 1) / (f * f);
+    };
+    Val5 beta;
+    for (int k = 0; k < 5; ++k) {
+      double s1 = integral[k][1];
+      s1 += integral[k][2];
+      s1 += integral[k][3];
+      s1 *= weight(1);
+      if (p >= 2) {
+        double s2 = integral[k][4];
+        for (int j = 5; j < 10; ++j) s2 += integral[k][j];
+        s2 *= weight(2);
+        s1 += s2;
+      }
+      beta[k] = s1;
+    }
+    return beta;
+  }
+  // weno.hpp:121-195 (weno::Lazy::Borrow + Reconstruct) of cell c from u.
+  void LazyReconstruct(int64_t c, double const *u, double *out) const {
+    auto const &cell = mesh->cells[c];
+    auto const &b = basis[c];
+    const int n_adj = (int)adj[c].size();
+    std::vector<std::vector<double>> proj(n_adj + 1, std::vector<double>(Stride(), 0.0));
+    Val5 my_avg = Average(c, u + c * Stride());
+    for (int ia = 0; ia < n_adj; ++ia) {
+      auto &pr = proj[ia];
+      for (size_t q = 0; q < cell.qp.size(); ++q) {  // Projection::Approximate of the neighbour
+        Val5 f = Value(adj[c][ia], u, cell.qp[q]);
+        double phi[10];
+        b.Values(cell.qp[q], phi);
+        for (int i = 0; i < n; ++i)
+          for (int k = 0; k < 5; ++k) {
+            double prod = f[k] * phi[i];
+            prod *= cell.qw[q];
+            pr[5 * i + k] += prod;
+          }
+      }
+      Val5 its_avg = Average(c, pr.data());
+      for (int k = 0; k < 5; ++k) {  // ProjectionWrapper::operator+=(Value)
+        double offset = my_avg[k] - its_avg[k];
+        offset /= b.s[0][0];
+        pr[k] += offset;
+      }
+    }
+    std::copy(u + c * Stride(), u + (c + 1) * Stride(), proj[n_adj].begin());
+    std::vector<Val5> weights(n_adj + 1);
+    for (int ia = 0; ia <= n_adj; ++ia)
+      for (int k = 0; k < 5; ++k) weights[ia][k] = weno_w0;
+    for (int k = 0; k < 5; ++k) { weights[n_adj][k] *= -n_adj; weights[n_adj][k] += 1.0; }
+    for (int ia = 0; ia <= n_adj; ++ia) {
+      Val5 beta = Smoothness(c, proj[ia].data());
+      for (int k = 0; k < 5; ++k) {
+        beta[k] += weno_eps;
+        beta[k] *= beta[k];
+        weights[ia][k] /= beta[k];
+      }
+    }
+    Val5 sum{ 0, 0, 0, 0, 0 };
+    for (auto const &w : weights) for (int k = 0; k < 5; ++k) sum[k] += w[k];
+    for (auto &w : weights) for (int k = 0; k < 5; ++k) w[k] /= sum[k];
+    for (int i = 0; i < n; ++i)
+      for (int k = 0; k < 5; ++k) {
+        double v = proj[n_adj][5 * i + k] * weights[n_adj][k];
+        for (int ia = 0; ia < n_adj; ++ia) v += proj[ia][5 * i + k] * weights[ia][k];
+        out[5 * i + k] = v;
+      }
+  }
+  // limiter/reconstruct.hpp:22-47: flag all cells, reconstruct the troubled
+  // ones from the old state, then write.  Returns the number of troubled cells.
+  int64_t Limit(double *u) {
+    const size_t nc = mesh->cells.size();
+    flags.assign(nc, 0);
+    if (limiter == 0) return 0;
+    std::vector<int64_t> troubled;
+    for (size_t c = 0; c < nc; ++c) {
+      flags[c] = limiter == 2 ? 1 : (IsNotSmooth(c, u) ? 1 : 0);
+      if (flags[c]) troubled.push_back(c);
+    }
+    std::vector<double> fresh(troubled.size() * Stride());
+    for (size_t t = 0; t < troubled.size(); ++t) LazyReconstruct(troubled[t], u, fresh.data() + t * Stride());
+    for (size_t t = 0; t < troubled.size(); ++t)
+      std::copy(fresh.begin() + t * Stride(), fresh.begin() + (t + 1) * Stride(), u + troubled[t] * Stride());
+    return (int64_t)troubled.size();
   }
 
   // fem.hpp:417-435: the first N quadrature points of each cell are sampled.

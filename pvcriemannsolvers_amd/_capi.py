@@ -20,6 +20,7 @@ NODES_LEGENDRE, NODES_LOBATTO = 0, 1
 RIEMANN_EXACT, RIEMANN_HLLC, RIEMANN_ROE = 0, 1, 2
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 FLAG_HOST_ONLY = 1
+LIMITER_NONE, LIMITER_WENO_FLAGGED, LIMITER_WENO_ALL = 0, 1, 2
 BC_SMART, BC_INVISCID_WALL, BC_SUPERSONIC_OUTLET, BC_SUPERSONIC_INLET = 0, 1, 2, 3
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -36,7 +37,8 @@ class MeshDesc(ctypes.Structure):
 class SchemeDesc(ctypes.Structure):
     _fields_ = [("scheme", ctypes.c_int32), ("degree", ctypes.c_int32), ("nodes", ctypes.c_int32),
                 ("riemann", ctypes.c_int32), ("precision", ctypes.c_int32), ("flags", ctypes.c_int32),
-                ("c_next", ctypes.c_double)]
+                ("c_next", ctypes.c_double), ("limiter", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("weno_w0", ctypes.c_double), ("weno_eps", ctypes.c_double)]
 
 
 class BcDesc(ctypes.Structure):
@@ -61,6 +63,7 @@ SIGNATURES = {
     "mcfd_approximate": [ctypes.c_void_p, _dp],
     "mcfd_face_count": [ctypes.c_void_p, _i64p, _i64p],
     "mcfd_faces": [ctypes.c_void_p, _i64p, _i64p, _i32p, _i32p, _i32p],
+    "mcfd_limiter_flags": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint8), _i64p],
     "mcfd_synchronize": [ctypes.c_void_p],
     "mcfd_time_rk3": [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32, _dp, _dp, _dp],
     "mcfd_batch_create": [ctypes.c_int32, ctypes.POINTER(MeshDesc), _i32p, ctypes.POINTER(SchemeDesc),

@@ -139,6 +139,8 @@ int mcfd_batch_create(int32_t n_cases, const mcfd_mesh_desc *meshes, const int32
   *out = nullptr;
   auto b = std::make_unique<mcfd_batch>();
   int rc = Guard([&] {
+    if (scheme->limiter != MCFD_LIMITER_NONE)
+      throw mcfd::Failure{ MCFD_ENOTSUP, "the batched form has no limiter" };
     if (scheme->scheme != MCFD_SCHEME_FR_GENERAL) throw mcfd::Failure{ MCFD_ENOTSUP, "batch implements FR_GENERAL" };
     b->n_cases = n_cases;
     b->riemann = scheme->riemann;

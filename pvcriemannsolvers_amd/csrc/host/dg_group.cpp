@@ -143,6 +143,12 @@ void DgGroup::Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &scheme,
   cell_type = mesh.cell_type;
   degree = scheme.degree;
   riemann = scheme.riemann;
+  if (scheme.limiter < MCFD_LIMITER_NONE || scheme.limiter > MCFD_LIMITER_WENO_ALL)
+    throw std::invalid_argument("unknown limiter");
+  limiter = scheme.limiter;
+  weno_w0 = scheme.weno_w0;
+  weno_eps = scheme.weno_eps;
+  if (limiter && !(weno_eps > 0)) throw std::invalid_argument("weno_eps must be positive");
   const bool hex = cell_type == MCFD_HEXA_8;
   const int npe = hex ? 8 : 4;
   n = degree == 1 ? 4 : 10;
@@ -313,6 +319,8 @@ void DgGroup::Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &scheme,
     for (int k = 0; k < 9; ++k) g[k] = j[k];
     g[9] = det;
     g[10] = std::cbrt(vol) * 0.5;  // mesh/part.hpp:230
+    for (int d = 0; d < 3; ++d) g[11 + d] = center[d];
+    g[14] = vol;
   }
 
   // ---- face records and per-cell face info
@@ -356,6 +364,17 @@ void DgGroup::Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &scheme,
   for (int64_t c = 0; c < n_cells; ++c)
     for (int lf = 0; lf < nf; ++lf)
       if (h_nbr[c * 6 + lf] == INT32_MIN) throw std::invalid_argument("cell face without a neighbour or boundary");
+
+  // ---- limiter neighbours (mesh/part.hpp:962-963): interior faces in order
+  h_adj.assign(n_cells * 6, -1);
+  h_nadj.assign(n_cells, 0);
+  for (int64_t f = 0; f < n_interior; ++f) {
+    const auto &face = faces[f];
+    if (limiter && (shift[f][0] != 0 || shift[f][1] != 0 || shift[f][2] != 0))
+      throw std::invalid_argument("the WENO limiter is not defined across periodic faces");
+    h_adj[face.holder * 6 + h_nadj[face.holder]++] = (int32_t)face.sharer;
+    h_adj[face.sharer * 6 + h_nadj[face.sharer]++] = (int32_t)face.holder;
+  }
 }
 
 void DgGroup::Upload(hipStream_t stream) {
@@ -371,6 +390,25 @@ void DgGroup::Upload(hipStream_t stream) {
   bc_kind.Upload(h_bc_kind, stream);
   h_tables.assign(1, tables);
   tables_d.Upload(h_tables, stream);
+  if (limiter) {
+    adj.Upload(h_adj, stream);
+    nadj.Upload(h_nadj, stream);
+    flags.Allocate(n_cells);
+    Check(hipMemsetAsync(flags.ptr, 0, n_cells, stream), "memset");
+    proj.Allocate(size_t(n_cells) * 7 * 5 * n);
+    beta.Allocate(size_t(n_cells) * 7 * 5);
+  }
+}
+
+void DgGroup::Limit(double *u, hipStream_t stream) {
+  if (!limiter) return;
+  Check(LaunchDgTaylorCoefficients(degree, u, basis.ptr, n_cells, ct.ptr, stream), "DG Taylor launch");
+  DgLimiterArgs a;
+  a.u = u; a.ct = ct.ptr; a.basis = basis.ptr; a.geo = geo.ptr;
+  a.adj = adj.ptr; a.n_adj = nadj.ptr; a.tables = tables_d.ptr;
+  a.flags = flags.ptr; a.proj = proj.ptr; a.beta = beta.ptr;
+  a.n_cells = n_cells; a.mode = limiter; a.w0 = weno_w0; a.eps = weno_eps;
+  Check(LaunchDgLimiter(cell_type, degree, a, stream), "DG limiter launch");
 }
 
 void DgGroup::Stage(int stage, double dt, hipStream_t stream) {
@@ -388,6 +426,7 @@ void DgGroup::Stage(int stage, double dt, hipStream_t stream) {
   a.n_cells = n_cells; a.dt = dt; a.stage = stage;
   Check(LaunchDgTaylorCoefficients(degree, a.us, basis.ptr, n_cells, ct.ptr, stream), "DG Taylor launch");
   Check(LaunchDgModalStage(cell_type, degree, riemann, a, stream), "DG stage launch");
+  if (stage > 0) Limit(a.out, stream);  // spatial/with_limiter.hpp:56-59 after each SetSolutionColumn
 }
 
 void DgGroup::TimeStepBits(unsigned long long *dt_min_bits, hipStream_t stream) {

@@ -34,6 +34,13 @@ struct DgGroup {
   DeviceArray<double> u0, u1, u2, res, ct, basis, geo, face_rec, bc_state;
   DeviceArray<int32_t> nbr, finfo, bc_kind;
   DeviceArray<DgTables> tables_d;
+  // WENO limiter (MCFD_LIMITER_*), applied to every state that is set
+  int limiter = 0;
+  double weno_w0 = 0.001, weno_eps = 1e-6;
+  std::vector<int32_t> h_adj, h_nadj;  // Cell::adj_cells_ in interior-face order, [cell][6]
+  DeviceArray<int32_t> adj, nadj;
+  DeviceArray<uint8_t> flags;
+  DeviceArray<double> proj, beta;
 
   void Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &scheme,
              const std::vector<int32_t> &bc_kinds, const std::vector<double> &bc_states);
@@ -42,6 +49,8 @@ struct DgGroup {
   size_t ColumnSize() const { return size_t(n_cells) * n * 5; }
   // Stage 0 residual, 1..3 SSP-RK3 stages (Taylor coefficients, then the stage kernel).
   void Stage(int stage, double dt, hipStream_t stream);
+  // Limiter pass over the state `u` (device column of this group).
+  void Limit(double *u, hipStream_t stream);
   // min over cells of length / max wave speed at the first N quadrature points.
   void TimeStepBits(unsigned long long *dt_min_bits, hipStream_t stream);
   // Coefficients of f given at the quadrature points (polynomial/projection.hpp:28-43).

@@ -67,6 +67,8 @@ int mcfd_create(const mcfd_mesh_desc *mesh, const mcfd_scheme_desc *scheme, cons
       throw mcfd::Failure{ MCFD_ENOTSUP, "FR_GENERAL is implemented on HEXA_8" };
     if (scheme->scheme != MCFD_SCHEME_FR_GENERAL && scheme->scheme != MCFD_SCHEME_DG_GENERAL)
       throw mcfd::Failure{ MCFD_ENOTSUP, "unknown scheme" };
+    if (scheme->scheme == MCFD_SCHEME_FR_GENERAL && scheme->limiter != MCFD_LIMITER_NONE)
+      throw mcfd::Failure{ MCFD_ENOTSUP, "limiters are implemented for DG_GENERAL" };
     std::vector<int32_t> kinds(n_bcs);
     std::vector<double> states(5 * n_bcs);
     for (int b = 0; b < n_bcs; ++b) {
@@ -124,6 +126,7 @@ int mcfd_set_solution_column(mcfd_solver *h, const double *column) {
   return Guard([&] {
     Check(hipMemcpyAsync(h->Solution(), column, h->ColumnSize() * sizeof(double),
                          hipMemcpyHostToDevice, h->stream), "set solution");
+    if (h->dg) h->modal.Limit(h->modal.u0.ptr, h->stream);  // spatial/with_limiter.hpp:56-59
     Check(hipStreamSynchronize(h->stream), "set solution sync");
   });
 }
@@ -234,6 +237,22 @@ int mcfd_faces(const mcfd_solver *h, int64_t *holder, int64_t *sharer, int32_t *
   }
   h->group.Faces(holder, sharer, holder_face, sharer_face, fp_ijk);
   return MCFD_OK;
+}
+
+int mcfd_limiter_flags(mcfd_solver *h, uint8_t *flags, int64_t *n_troubled) {
+  MCFD_REQUIRE_DEVICE(h);
+  if (!h->dg || !h->modal.limiter) return Fail(MCFD_ESTATE, "the handle has no limiter");
+  return Guard([&] {
+    std::vector<uint8_t> f(h->modal.n_cells);
+    Check(hipMemcpyAsync(f.data(), h->modal.flags.ptr, f.size(), hipMemcpyDeviceToHost, h->stream), "flags");
+    Check(hipStreamSynchronize(h->stream), "flags sync");
+    if (flags) std::copy(f.begin(), f.end(), flags);
+    if (n_troubled) {
+      int64_t n = 0;
+      for (auto v : f) n += v;
+      *n_troubled = n;
+    }
+  });
 }
 
 int mcfd_synchronize(mcfd_solver *h) {

@@ -258,6 +258,191 @@ __global__ __launch_bounds__(256) void DgTimeStepKernel(const double *ct, const 
   if (threadIdx.x == 0) atomicMin(dt_min_bits, (unsigned long long)__double_as_longlong(s_min[0]));
 }
 
+// ---------------------------------------------------------------- WENO limiter
+constexpr int kLimiterSlots = 7;  // up to 6 neighbours, then the cell itself
+
+// u(c_self) from a cell's Taylor coefficients: Ct T(centre_self - centre_cell).
+template <int P>
+__device__ __forceinline__ State ValueAtPoint(const double *ct, const double *geo_cell, const double *x) {
+  const double y[3] = { x[0] - geo_cell[11], x[1] - geo_cell[12], x[2] - geo_cell[13] };
+  return EvalTaylor<P>(ct, y);
+}
+
+// limiter/weno.hpp:230-258 (weno::Eigen::IsNotSmooth) on rho and E.
+template <int P>
+__global__ __launch_bounds__(256) void DgWenoFlagsKernel(DgLimiterArgs a) {
+  constexpr int N = P == 1 ? 4 : 10, PACKED = N * (N + 1) / 2;
+  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= a.n_cells) return;
+  if (a.mode == 2) {
+    a.flags[c] = 1;
+    return;
+  }
+  const double *gc = a.geo + c * kDgGeoStride;
+  const double s00 = a.basis[c * PACKED];
+  const double *cf = a.u + c * 5 * N;
+  const double *ct = a.ct + c * 5 * N;
+  double max_abs0 = fmax(1e-9, fabs(cf[0] * s00)), max_abs4 = fmax(1e-9, fabs(cf[4] * s00));
+  double sum0 = 0.0, sum4 = 0.0;
+  const double mine0 = ct[0], mine4 = ct[4];  // u(centre) = Ct T(0)
+  const int na = a.n_adj[c];
+  for (int i = 0; i < na; ++i) {
+    const int64_t o = a.adj[c * 6 + i];
+    const State theirs = ValueAtPoint<P>(a.ct + o * 5 * N, a.geo + o * kDgGeoStride, gc + 11);
+    const double so = a.basis[o * PACKED];
+    sum0 += fabs(mine0 - theirs.v[0]);
+    sum4 += fabs(mine4 - theirs.v[4]);
+    max_abs0 = fmax(max_abs0, fabs(a.u[o * 5 * N] * so));
+    max_abs4 = fmax(max_abs4, fabs(a.u[o * 5 * N + 4] * so));
+  }
+  double divisor = pow(gc[14], (P + 1.0) / 2.0 / 3);
+  divisor *= na;
+  const double reference = P < 3 ? 1.0 : 3.0;
+  const bool rough = sum0 / max_abs0 / divisor > reference || sum4 / max_abs4 / divisor > reference;
+  a.flags[c] = rough ? 1 : 0;
+}
+
+// weno.hpp:74-95 (GetSmoothness) of component k of coefficients `pr` ([N][5])
+// on the basis `s` of a cell with geometry `gc`.
+template <bool kHex, int P>
+__device__ double Smoothness(const double *pr, int k, const double *s, const double *gc, const DgTables *tb) {
+  constexpr int N = P == 1 ? 4 : 10, NQ = DgShape<kHex, P>::kNQ;
+  double tc[10];  // Taylor coefficients of component k: (C S)[k][j]
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    double acc = 0.0;
+    for (int i = j; i < N; ++i) acc = __fma_rn(pr[5 * i + k], s[i * (i + 1) / 2 + j], acc);
+    tc[j] = acc;
+  }
+  double integ[10];
+#pragma unroll
+  for (int j = 0; j < N; ++j) integ[j] = 0.0;
+  for (int q = 0; q < NQ; ++q) {
+    double y[3];
+    Offset(gc, tb->vol_eta[q], y);
+    const double w = tb->vol_w[q] * gc[9];
+    double pdv[10];  // taylor.hpp:241-281 (GetPartialDerivatives)
+    pdv[0] = 0.0;
+#pragma unroll
+    for (int j = 1; j < N; ++j) pdv[j] = tc[j];
+    if constexpr (P >= 2) {
+      pdv[1] += tc[4] * (2 * y[0]); pdv[1] += tc[5] * y[1]; pdv[1] += tc[6] * y[2];
+      pdv[2] += tc[5] * y[0]; pdv[2] += tc[7] * (2 * y[1]); pdv[2] += tc[8] * y[2];
+      pdv[3] += tc[6] * y[0]; pdv[3] += tc[8] * y[1]; pdv[3] += tc[9] * (2 * y[2]);
+      pdv[4] += tc[4]; pdv[7] += tc[7]; pdv[9] += tc[9];
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) integ[j] += pdv[j] * pdv[j] * w;
+  }
+  const double s00 = s[0];
+  const double volume = 1 / (s00 * s00);  // OrthoNormal::Measure
+  double beta = integ[1];
+  beta += integ[2];
+  beta += integ[3];
+  beta *= pow(volume, 2.0 / 3 - 1);
+  if constexpr (P >= 2) {
+    double b2 = integ[4];
+#pragma unroll
+    for (int j = 5; j < 10; ++j) b2 += integ[j];
+    b2 *= pow(volume, 4.0 / 3 - 1) / 4.0;
+    beta += b2;
+  }
+  return beta;
+}
+
+// weno.hpp:121-155 (Lazy::Borrow): one thread per (troubled cell, slot).
+// Slot i < n_adj projects neighbour i onto the cell's basis and shifts its
+// average onto the cell's; slot n_adj is the cell's own expansion.
+template <bool kHex, int P>
+__global__ __launch_bounds__(256) void DgWenoProjectKernel(DgLimiterArgs a) {
+  constexpr int N = P == 1 ? 4 : 10, PACKED = N * (N + 1) / 2, NQ = DgShape<kHex, P>::kNQ;
+  const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t c = w / kLimiterSlots;
+  const int slot = (int)(w - c * kLimiterSlots);
+  if (c >= a.n_cells || !a.flags[c]) return;
+  const int na = a.n_adj[c];
+  if (slot > na) return;
+  const double *gc = a.geo + c * kDgGeoStride;
+  const double *s = a.basis + c * PACKED;
+  const DgTables *tb = a.tables;
+  double *pr = a.proj + (c * kLimiterSlots + slot) * 5 * N;
+  double acc[10][5];
+  if (slot == na) {
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int k = 0; k < 5; ++k) acc[i][k] = a.u[c * 5 * N + 5 * i + k];
+  } else {
+    const int64_t o = a.adj[c * 6 + slot];
+    const double *go = a.geo + o * kDgGeoStride;
+    const double shift[3] = { gc[11] - go[11], gc[12] - go[12], gc[13] - go[13] };
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int k = 0; k < 5; ++k) acc[i][k] = 0.0;
+    for (int q = 0; q < NQ; ++q) {  // projection.hpp:28-43 (Project)
+      double y[3];
+      Offset(gc, tb->vol_eta[q], y);
+      const double yo[3] = { y[0] + shift[0], y[1] + shift[1], y[2] + shift[2] };
+      const State f = EvalTaylor<P>(a.ct + o * 5 * N, yo);
+      double t[10];
+      t[0] = 1.0; t[1] = y[0]; t[2] = y[1]; t[3] = y[2];
+      if constexpr (P >= 2) {
+        t[4] = y[0] * y[0]; t[5] = y[0] * y[1]; t[6] = y[0] * y[2];
+        t[7] = y[1] * y[1]; t[8] = y[1] * y[2]; t[9] = y[2] * y[2];
+      }
+      const double wq = tb->vol_w[q] * gc[9];
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        double phi = 0.0;
+        for (int j = 0; j <= i; ++j) phi = __fma_rn(s[i * (i + 1) / 2 + j], t[j], phi);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) acc[i][k] += (f.v[k] * phi) * wq;
+      }
+    }
+    const double s00 = s[0];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {  // ProjectionWrapper::operator+=(my average - its average)
+      const double offset = (a.u[c * 5 * N + k] * s00 - acc[0][k] * s00) / s00;
+      acc[0][k] += offset;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int k = 0; k < 5; ++k) pr[5 * i + k] = acc[i][k];
+  for (int k = 0; k < 5; ++k)
+    a.beta[(c * kLimiterSlots + slot) * 5 + k] = Smoothness<kHex, P>(pr, k, s, gc, tb);
+}
+
+// weno.hpp:157-194 (Lazy::Reconstruct): nonlinear weights and the weighted
+// sum, one thread per (troubled cell, basis function, component); written
+// over the state only after every projection has been read.
+template <int P>
+__global__ __launch_bounds__(256) void DgWenoCombineKernel(DgLimiterArgs a) {
+  constexpr int N = P == 1 ? 4 : 10;
+  const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t c = w / (5 * N);
+  const int rem = (int)(w - c * 5 * N), k = rem % 5;
+  if (c >= a.n_cells || !a.flags[c]) return;
+  const int na = a.n_adj[c];
+  double wt[kLimiterSlots];
+  for (int i = 0; i <= na; ++i) {
+    double wi = a.w0;
+    if (i == na) { wi *= -na; wi += 1.0; }
+    double b = a.beta[(c * kLimiterSlots + i) * 5 + k];
+    b += a.eps;
+    b *= b;
+    wt[i] = wi / b;
+  }
+  double sum = 0.0;
+  for (int i = 0; i <= na; ++i) sum += wt[i];
+  const double *pr = a.proj + c * kLimiterSlots * 5 * N;
+  double v = pr[na * 5 * N + rem] * (wt[na] / sum);
+  for (int i = 0; i < na; ++i) v += pr[i * 5 * N + rem] * (wt[i] / sum);
+  a.u[c * 5 * N + rem] = v;
+}
+
 template <bool kHex, int P, int R>
 hipError_t LaunchStage(const DgModalArgs &a, hipStream_t stream) {
   constexpr int G = DgShape<kHex, P>::kG;
@@ -303,6 +488,36 @@ hipError_t LaunchDgModalStage(int cell_type, int degree, int riemann, const DgMo
   if (!hex && degree == 1) return LaunchStageR<false, 1>(riemann, a, stream);
   if (!hex && degree == 2) return LaunchStageR<false, 2>(riemann, a, stream);
   return hipErrorInvalidValue;
+}
+
+hipError_t LaunchDgLimiter(int cell_type, int degree, const DgLimiterArgs &a, hipStream_t stream) {
+  if (a.n_cells < 1 || a.mode == 0) return hipSuccess;
+  const bool hex = cell_type == 0;
+  const int n = degree == 1 ? 4 : 10;
+  const int64_t b_flags = (a.n_cells + 255) / 256;
+  const int64_t b_proj = (a.n_cells * kLimiterSlots + 255) / 256;
+  const int64_t b_comb = (a.n_cells * 5 * n + 255) / 256;
+  if (b_comb > 0x7fffffff || b_proj > 0x7fffffff) return hipErrorInvalidValue;
+  if (degree == 1) {
+    hipLaunchKernelGGL(DgWenoFlagsKernel<1>, dim3((unsigned)b_flags), dim3(256), 0, stream, a);
+    if (hex) {
+      hipLaunchKernelGGL((DgWenoProjectKernel<true, 1>), dim3((unsigned)b_proj), dim3(256), 0, stream, a);
+    } else {
+      hipLaunchKernelGGL((DgWenoProjectKernel<false, 1>), dim3((unsigned)b_proj), dim3(256), 0, stream, a);
+    }
+    hipLaunchKernelGGL(DgWenoCombineKernel<1>, dim3((unsigned)b_comb), dim3(256), 0, stream, a);
+  } else if (degree == 2) {
+    hipLaunchKernelGGL(DgWenoFlagsKernel<2>, dim3((unsigned)b_flags), dim3(256), 0, stream, a);
+    if (hex) {
+      hipLaunchKernelGGL((DgWenoProjectKernel<true, 2>), dim3((unsigned)b_proj), dim3(256), 0, stream, a);
+    } else {
+      hipLaunchKernelGGL((DgWenoProjectKernel<false, 2>), dim3((unsigned)b_proj), dim3(256), 0, stream, a);
+    }
+    hipLaunchKernelGGL(DgWenoCombineKernel<2>, dim3((unsigned)b_comb), dim3(256), 0, stream, a);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 hipError_t LaunchDgTimeStep(int cell_type, int degree, const double *ct, const double *geo, const DgTables *tables,

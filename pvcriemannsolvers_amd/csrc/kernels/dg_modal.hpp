@@ -22,7 +22,7 @@ namespace mcfd {
 constexpr int kDgMaxVolumePoints = 27;  // 3^3 Gauss points (hexahedra, p = 2)
 constexpr int kDgMaxFacePoints = 9;     // 3^2 Gauss points (quadrangles, p = 2)
 constexpr int kDgMaxOrient = 8;         // dihedral group of the square
-constexpr int kDgGeoStride = 12;        // J rows (9), |det J| (1), length (1), pad
+constexpr int kDgGeoStride = 16;        // J rows (9), |det J| (1), length (1), centre (3), volume (1), pad
 constexpr int kDgFaceStride = 8;        // n (3), t (3), area factor (1), pad
 
 // Reference-element tables of one (cell type, degree); lives in device memory.
@@ -59,6 +59,25 @@ hipError_t LaunchDgTaylorCoefficients(int degree, const double *us, const double
 // One residual / RK-stage pass (reads a.ct, which must hold the Taylor
 // coefficients of a.us).
 hipError_t LaunchDgModalStage(int cell_type, int degree, int riemann, const DgModalArgs &a, hipStream_t stream);
+// WENO limiter (limiter/weno.hpp, limiter/reconstruct.hpp) over a state
+// column `u` whose Taylor coefficients are in `ct`.
+struct DgLimiterArgs {
+  double *u;              // state, reconstructed in place for troubled cells
+  const double *ct;       // Taylor coefficients of u
+  const double *basis;    // as DgModalArgs
+  const double *geo;
+  const int32_t *adj;     // [cell][6] neighbours in interior-face order (Cell::adj_cells_)
+  const int32_t *n_adj;   // [cell]
+  const DgTables *tables;
+  uint8_t *flags;         // [cell]
+  double *proj;           // scratch [cell][7][N][5]: neighbour projections, then the cell's own
+  double *beta;           // scratch [cell][7][5]: their smoothness indicators
+  int64_t n_cells;
+  int32_t mode;           // 1 flagged by weno::Eigen::IsNotSmooth, 2 all cells
+  double w0, eps;
+};
+hipError_t LaunchDgLimiter(int cell_type, int degree, const DgLimiterArgs &a, hipStream_t stream);
+
 // min over cells of length / max_{q < N} (|u| + c)(x_q), as ordered bits of a
 // positive double folded into *dt_min_bits with atomicMin.
 hipError_t LaunchDgTimeStep(int cell_type, int degree, const double *ct, const double *geo,

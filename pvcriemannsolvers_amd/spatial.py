@@ -43,9 +43,12 @@ class FiniteElement(System):
     _BC = {"smart": C.BC_SMART, "inviscid_wall": C.BC_INVISCID_WALL,
            "supersonic_outlet": C.BC_SUPERSONIC_OUTLET, "supersonic_inlet": C.BC_SUPERSONIC_INLET}
 
+    _LIMITER = {"none": C.LIMITER_NONE, "weno": C.LIMITER_WENO_FLAGGED, "weno_all": C.LIMITER_WENO_ALL}
+
     def __init__(self, mesh: dict, degree: int, scheme: str = "fr", riemann: str = "roe",
                  nodes: str = "legendre", c_next: float = 1.0, bcs=(), precision: str = "fp64",
-                 host_only: bool = False):
+                 host_only: bool = False, limiter: str = "none", weno_w0: float = 0.001,
+                 weno_eps: float = 1e-6):
         lib = C.load()
         self._keep = [np.ascontiguousarray(mesh["coords"], np.float64),
                       np.ascontiguousarray(mesh["cells"], np.int64),
@@ -67,6 +70,8 @@ class FiniteElement(System):
         sd.precision = C.PRECISION_MIXED if precision == "mixed" else C.PRECISION_FP64
         sd.flags = C.FLAG_HOST_ONLY if host_only else 0
         sd.c_next = c_next
+        sd.limiter = self._LIMITER[limiter]
+        sd.weno_w0, sd.weno_eps = float(weno_w0), float(weno_eps)
         bc_arr = (C.BcDesc * max(1, len(bcs)))()
         for i, (kind, state) in enumerate(bcs):
             bc_arr[i].kind = self._BC[kind]
@@ -107,6 +112,14 @@ class FiniteElement(System):
         out = np.empty(self.column_size)
         C.check(C.load().mcfd_get_residual_column(self._h, C.ptr(out, C._dp)))
         return out
+
+    def limiter_flags(self):
+        """Troubled-cell flags of the last limiter pass (limiter/reconstruct.hpp:29-34)."""
+        flags = np.zeros(self.n_cells, np.uint8)
+        n = ctypes.c_int64()
+        C.check(C.load().mcfd_limiter_flags(self._h, flags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                            ctypes.byref(n)))
+        return flags
 
     # ---- fused device paths
     def rk3_update(self, t_curr, dt):

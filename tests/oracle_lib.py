@@ -301,3 +301,34 @@ class DgOracle:
         u, x, v = f64(u), f64(x), np.zeros(5)
         load().oracle_dg_value(self.h, _ptr(u, _dp), int(cell), _ptr(x, _dp), _ptr(v, _dp))
         return v
+
+    # ---- WENO limiter (limiter/weno.hpp restated)
+    def set_limiter(self, mode: int, w0: float = 0.001, eps: float = 1e-6):
+        lib = load()
+        lib.oracle_dg_set_limiter.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        lib.oracle_dg_set_limiter(self.h, int(mode), float(w0), float(eps))
+
+    def limit(self, u):
+        """Applies the limiter (as SetSolutionColumn does); returns (new u, flags)."""
+        lib = load()
+        lib.oracle_dg_limit.restype = ctypes.c_int64
+        lib.oracle_dg_limit.argtypes = [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_uint8)]
+        u = f64(u).copy()
+        flags = np.zeros(self.n_cells, np.uint8)
+        lib.oracle_dg_limit(self.h, _ptr(u, _dp), flags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+        return u, flags
+
+    def last_flags(self):
+        """Flags of the oracle's last limiter pass (the last RK stage after rk3)."""
+        lib = load()
+        lib.oracle_dg_last_flags.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint8)]
+        flags = np.zeros(self.n_cells, np.uint8)
+        lib.oracle_dg_last_flags(self.h, flags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+        return flags
+
+    def smoothness(self, cell, coeff):
+        lib = load()
+        lib.oracle_dg_smoothness.argtypes = [ctypes.c_void_p, ctypes.c_int64, _dp, _dp]
+        coeff, beta = f64(coeff), np.zeros(5)
+        lib.oracle_dg_smoothness(self.h, int(cell), _ptr(coeff, _dp), _ptr(beta, _dp))
+        return beta

@@ -156,3 +156,54 @@ def test_free_stream_and_conservation_on_device():
     total = (r[:, 0, :] * np.sqrt(vol)[:, None]).sum(axis=0)
     assert abs(total[0]) < 1e-13 and abs(total[4]) < 1e-13
     g.close()
+
+
+# ---------------------------------------------------------------- WENO limiter
+def quadrant_state(x, seed=21):
+    """Four constant quadrants of [0, 1]^2 (the 2-D Riemann problem of riemann2d.cpp), seeded."""
+    rng = np.random.default_rng(seed)
+    rho, p = rng.uniform(0.1, 1.5, 4), rng.uniform(0.03, 1.5, 4)
+    vel = rng.uniform(-1, 1, (4, 2))
+    q = (x[..., 0] > 0.5).astype(int) + 2 * (x[..., 1] > 0.5).astype(int)
+    return cons(rho[q], (vel[q, 0], vel[q, 1], 0.0 * q), p[q])
+
+
+@pytest.mark.parametrize("mode", ["weno", "weno_all"])
+@pytest.mark.parametrize("name", sorted(MESHES))
+def test_limiter_flags_and_state_match_oracle(mode, name):
+    mesh = MESHES[name]()
+    bcs = [("inviscid_wall", None)] * 6
+    o = ol.DgOracle(mesh, degree=2, riemann=1, bcs=[(1, [0] * 5)] * 6)
+    o.set_limiter({"weno": 1, "weno_all": 2}[mode])
+    g = spatial.FiniteElement(mesh, degree=2, scheme="dg", riemann="hllc", bcs=bcs, limiter=mode)
+    x = o.cells()["xyz"]
+    vals = quadrant_state(x)
+    u = o.approximate(vals)
+    g.set_solution_column(u)  # SetSolutionColumn runs the limiter
+    v, flags = o.limit(u)
+    np.testing.assert_array_equal(g.limiter_flags(), flags)
+    assert flags.any()
+    assert rel(g.get_solution_column(), v) <= 1e-12
+    g.close()
+
+
+def test_limited_rk3_on_tets_matches_oracle():
+    """4-quadrant problem on jittered tetrahedra, HLLC, WENO on flagged cells, 3 SSP-RK3 steps."""
+    mesh = meshgen.box_tet(8, 8, 2, hi=(1.0, 1.0, 0.25), jitter=0.15, seed=5)
+    bcs = [("inviscid_wall", None)] * 6
+    o = ol.DgOracle(mesh, degree=2, riemann=1, bcs=[(1, [0] * 5)] * 6)
+    o.set_limiter(1)
+    g = spatial.FiniteElement(mesh, degree=2, scheme="dg", riemann="hllc", bcs=bcs, limiter="weno")
+    vals = quadrant_state(g.sample_points())
+    g.approximate(vals)
+    u, _ = o.limit(o.approximate(vals))
+    assert rel(g.get_solution_column(), u) <= 1e-12
+    for s in range(3):
+        dt = o.time_step(u, 0.1)
+        assert abs(g.get_time_step(0.1) - dt) <= 1e-12 * dt
+        g.rk3_update(0.0, dt)
+        u = o.rk3(u, dt, 1)
+        # flags of the last stage's limiter pass, on both sides
+        np.testing.assert_array_equal(g.limiter_flags(), o.last_flags())
+    assert rel(g.get_solution_column(), u) <= 1e-12
+    g.close()

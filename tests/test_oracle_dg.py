@@ -234,3 +234,101 @@ def test_time_step_samples_first_n_points():
     expect = 0.3 * np.min(c["lengths"] / np.maximum(1e-5, speeds.max(axis=1)))
     assert abs(o.time_step(u, 0.3) - expect) <= 1e-14 * expect
     assert np.allclose(c["lengths"], np.cbrt(c["w"].sum(axis=1)) / 2, rtol=1e-15)
+
+
+# ---------------------------------------------------------------- WENO limiter
+def cube_cell():
+    return meshgen.box_hex(1, 1, 1, lo=(-1.0, -1.0, -1.0), hi=(1.0, 1.0, 1.0))
+
+
+def test_smoothness_closed_forms():
+    """test/limiter/weno.cpp:45-88 on the cube [-1, 1]^3: beta of x, x^2, y^2, xy, yz."""
+    o = ol.DgOracle(cube_cell(), degree=2, riemann=HLLC, bcs=[(OUTLET, [0] * 5)] * 6)
+    c = o.cells()
+    x = c["xyz"]
+    vals = np.stack([x[..., 0], x[..., 0] ** 2, x[..., 1] ** 2, x[..., 0] * x[..., 1], x[..., 1] * x[..., 2]], -1)
+    u = o.approximate(vals).reshape(o.n, 5)
+    beta = o.smoothness(0, u)
+    volume = 8.0
+    w1 = volume ** (2 / 3 - 1)
+    w2 = volume ** (4 / 3 - 1) / 4
+    np.testing.assert_allclose(beta[0], volume * w1, rtol=1e-13)
+    np.testing.assert_allclose(beta[1], 32 / 3 * w1 + 32 * w2, rtol=1e-13)
+    np.testing.assert_allclose(beta[2], 32 / 3 * w1 + 32 * w2, rtol=1e-13)
+    np.testing.assert_allclose(beta[3], 16 / 3 * w1 + 8 * w2, rtol=1e-13)
+    np.testing.assert_allclose(beta[4], 16 / 3 * w1 + 8 * w2, rtol=1e-13)
+
+
+def rough_state(x, seed=9):
+    rng = np.random.default_rng(seed)
+    jump = (x[..., 0] + 0.3 * x[..., 1] > 0.55)
+    rho = np.where(jump, 0.2, 1.0) + 0.05 * np.sin(5 * x[..., 1])
+    p = np.where(jump, 0.15, 1.0)
+    u = cons(rho, (0.1, -0.1, 0.05), p)
+    return u * (1 + 1e-3 * rng.standard_normal(u.shape[:-1]))[..., None]
+
+
+@pytest.mark.parametrize("name", sorted(MESHES))
+def test_detector_matches_its_definition(name):
+    """weno.hpp:230-258 recomputed from point values and averages on every cell."""
+    mesh = MESHES[name]()
+    o = ol.DgOracle(mesh, degree=2, riemann=HLLC, bcs=[(WALL, [0] * 5)] * 6)
+    o.set_limiter(1)
+    c = o.cells()
+    u = o.approximate(rough_state(c["xyz"]))
+    _, flags = o.limit(u)
+    f = o.faces()
+    adj = [[] for _ in range(o.n_cells)]
+    for k in range(f["n_interior"]):
+        adj[f["holder"][k]].append(f["sharer"][k])
+        adj[f["sharer"][k]].append(f["holder"][k])
+    coef = u.reshape(o.n_cells, o.n, 5)
+    avg = coef[:, 0, :] * c["s"][:, 0, 0][:, None]
+    vol = c["w"].sum(axis=1)
+    expect = np.zeros(o.n_cells, np.uint8)
+    for cell in range(o.n_cells):
+        mine = o.value(u, cell, c["centers"][cell])
+        top = np.maximum(1e-9, np.abs(avg[cell]))
+        total = np.zeros(5)
+        for a in adj[cell]:
+            total += np.abs(mine - o.value(u, a, c["centers"][cell]))
+            top = np.maximum(top, np.abs(avg[a]))
+        s = total / top / (np.sqrt(vol[cell]) * len(adj[cell]))
+        expect[cell] = s[0] > 1 or s[4] > 1
+    np.testing.assert_array_equal(flags, expect)
+    assert 0 < flags.sum() < o.n_cells
+
+
+@pytest.mark.parametrize("name", sorted(MESHES))
+def test_lazy_reconstruction_properties(name):
+    """Averages kept (test/limiter/weno.cpp:301-305); linear data unchanged; a smooth
+    cell among rough neighbours keeps its own polynomial (the central weight wins)."""
+    mesh = MESHES[name]()
+    o = ol.DgOracle(mesh, degree=2, riemann=HLLC, bcs=[(WALL, [0] * 5)] * 6)
+    o.set_limiter(2)
+    c = o.cells()
+    u = o.approximate(rough_state(c["xyz"]))
+    v, flags = o.limit(u)
+    assert flags.all()
+    cu, cv = u.reshape(o.n_cells, o.n, 5), v.reshape(o.n_cells, o.n, 5)
+    np.testing.assert_allclose(cv[:, 0, :], cu[:, 0, :], rtol=1e-13, atol=1e-13 * np.abs(cu[:, 0, :]).max())
+    # globally linear data: every borrowed projection equals the cell's own one
+    x = c["xyz"]
+    lin = cons(1 + 0.2 * x[..., 0] - 0.1 * x[..., 1], (0.1, 0.2, 0.0), 1 + 0.3 * x[..., 2])
+    u = o.approximate(lin)
+    v, _ = o.limit(u)
+    np.testing.assert_allclose(v, u, rtol=0, atol=1e-12 * np.abs(u).max())
+    o.set_limiter(1)
+    _, flags = o.limit(u)
+    assert not flags.any()
+    # one smooth cell among rough neighbours
+    o.set_limiter(2)
+    rng = np.random.default_rng(3)
+    u = o.approximate(lin).reshape(o.n_cells, o.n, 5)
+    noisy = u.copy()
+    noisy[:, 1:, :] += 0.3 * rng.standard_normal(noisy[:, 1:, :].shape)
+    target = 0
+    noisy[target] = u[target]
+    v, _ = o.limit(noisy.reshape(-1))
+    v = v.reshape(o.n_cells, o.n, 5)
+    assert np.abs(v[target] - u[target]).max() < 1e-3 * np.abs(u[target]).max()
