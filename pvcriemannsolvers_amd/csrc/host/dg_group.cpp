@@ -379,7 +379,9 @@ void DgGroup::Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &scheme,
 
 void DgGroup::Upload(hipStream_t stream) {
   const size_t cs = ColumnSize();
-  u0.Allocate(cs); u1.Allocate(cs); u2.Allocate(cs); res.Allocate(cs); ct.Allocate(cs);
+  u0.Allocate(cs); u1.Allocate(cs); u2.Allocate(cs); res.Allocate(cs);
+  ct0.Allocate(cs); ct1.Allocate(cs); ct2.Allocate(cs);
+  Check(hipMemsetAsync(ct0.ptr, 0, cs * sizeof(double), stream), "memset");
   Check(hipMemsetAsync(u0.ptr, 0, cs * sizeof(double), stream), "memset");
   basis.Upload(h_basis, stream);
   geo.Upload(h_geo, stream);
@@ -400,38 +402,40 @@ void DgGroup::Upload(hipStream_t stream) {
   }
 }
 
-void DgGroup::Limit(double *u, hipStream_t stream) {
+void DgGroup::Limit(double *u, double *ct, hipStream_t stream) {
   if (!limiter) return;
-  Check(LaunchDgTaylorCoefficients(degree, u, basis.ptr, n_cells, ct.ptr, stream), "DG Taylor launch");
   DgLimiterArgs a;
-  a.u = u; a.ct = ct.ptr; a.basis = basis.ptr; a.geo = geo.ptr;
+  a.u = u; a.ct = ct; a.basis = basis.ptr; a.geo = geo.ptr;
   a.adj = adj.ptr; a.n_adj = nadj.ptr; a.tables = tables_d.ptr;
   a.flags = flags.ptr; a.proj = proj.ptr; a.beta = beta.ptr;
   a.n_cells = n_cells; a.mode = limiter; a.w0 = weno_w0; a.eps = weno_eps;
   Check(LaunchDgLimiter(cell_type, degree, a, stream), "DG limiter launch");
 }
 
+void DgGroup::SolutionChanged(hipStream_t stream) {
+  Check(LaunchDgTaylorCoefficients(degree, u0.ptr, basis.ptr, n_cells, ct0.ptr, stream), "DG Taylor launch");
+  Limit(u0.ptr, ct0.ptr, stream);  // spatial/with_limiter.hpp:56-59
+}
+
 void DgGroup::Stage(int stage, double dt, hipStream_t stream) {
   DgModalArgs a;
   a.u0 = u0.ptr;
   switch (stage) {
-  case 0: a.us = u0.ptr; a.out = res.ptr; break;
-  case 1: a.us = u0.ptr; a.out = u1.ptr; break;
-  case 2: a.us = u1.ptr; a.out = u2.ptr; break;
-  default: a.us = u2.ptr; a.out = u0.ptr; break;
+  case 0: a.us = u0.ptr; a.ct = ct0.ptr; a.out = res.ptr; a.ct_out = nullptr; break;
+  case 1: a.us = u0.ptr; a.ct = ct0.ptr; a.out = u1.ptr; a.ct_out = ct1.ptr; break;
+  case 2: a.us = u1.ptr; a.ct = ct1.ptr; a.out = u2.ptr; a.ct_out = ct2.ptr; break;
+  default: a.us = u2.ptr; a.ct = ct2.ptr; a.out = u0.ptr; a.ct_out = ct0.ptr; break;
   }
-  a.ct = ct.ptr; a.basis = basis.ptr; a.geo = geo.ptr;
+  a.basis = basis.ptr; a.geo = geo.ptr;
   a.nbr = nbr.ptr; a.finfo = finfo.ptr; a.faces = face_rec.ptr;
   a.bc_kind = bc_kind.ptr; a.bc_state = bc_state.ptr; a.tables = tables_d.ptr;
   a.n_cells = n_cells; a.dt = dt; a.stage = stage;
-  Check(LaunchDgTaylorCoefficients(degree, a.us, basis.ptr, n_cells, ct.ptr, stream), "DG Taylor launch");
   Check(LaunchDgModalStage(cell_type, degree, riemann, a, stream), "DG stage launch");
-  if (stage > 0) Limit(a.out, stream);  // spatial/with_limiter.hpp:56-59 after each SetSolutionColumn
+  if (stage > 0) Limit(a.out, a.ct_out, stream);  // spatial/with_limiter.hpp:56-59 after each SetSolutionColumn
 }
 
 void DgGroup::TimeStepBits(unsigned long long *dt_min_bits, hipStream_t stream) {
-  Check(LaunchDgTaylorCoefficients(degree, u0.ptr, basis.ptr, n_cells, ct.ptr, stream), "DG Taylor launch");
-  Check(LaunchDgTimeStep(cell_type, degree, ct.ptr, geo.ptr, tables_d.ptr, n_cells, dt_min_bits, stream),
+  Check(LaunchDgTimeStep(cell_type, degree, ct0.ptr, geo.ptr, tables_d.ptr, n_cells, dt_min_bits, stream),
         "DG time-step launch");
 }
 

@@ -31,7 +31,8 @@ struct DgGroup {
   std::vector<double> h_basis, h_geo, h_faces, h_bc_state;
   std::vector<int32_t> h_nbr, h_finfo, h_bc_kind;
   std::vector<double> qp, qw;          // global quadrature points [cell][q][3] and weights [cell][q]
-  DeviceArray<double> u0, u1, u2, res, ct, basis, geo, face_rec, bc_state;
+  DeviceArray<double> u0, u1, u2, res, basis, geo, face_rec, bc_state;
+  DeviceArray<double> ct0, ct1, ct2;  // Taylor coefficients of u0, u1, u2 (kept current)
   DeviceArray<int32_t> nbr, finfo, bc_kind;
   DeviceArray<DgTables> tables_d;
   // WENO limiter (MCFD_LIMITER_*), applied to every state that is set
@@ -49,8 +50,10 @@ struct DgGroup {
   size_t ColumnSize() const { return size_t(n_cells) * n * 5; }
   // Stage 0 residual, 1..3 SSP-RK3 stages (Taylor coefficients, then the stage kernel).
   void Stage(int stage, double dt, hipStream_t stream);
-  // Limiter pass over the state `u` (device column of this group).
-  void Limit(double *u, hipStream_t stream);
+  // Limiter pass over the state `u` with Taylor coefficients `ct`.
+  void Limit(double *u, double *ct, hipStream_t stream);
+  // The state u0 was set from outside: form its Taylor coefficients and limit it.
+  void SolutionChanged(hipStream_t stream);
   // min over cells of length / max wave speed at the first N quadrature points.
   void TimeStepBits(unsigned long long *dt_min_bits, hipStream_t stream);
   // Coefficients of f given at the quadrature points (polynomial/projection.hpp:28-43).

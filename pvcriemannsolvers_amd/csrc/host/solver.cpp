@@ -126,7 +126,7 @@ int mcfd_set_solution_column(mcfd_solver *h, const double *column) {
   return Guard([&] {
     Check(hipMemcpyAsync(h->Solution(), column, h->ColumnSize() * sizeof(double),
                          hipMemcpyHostToDevice, h->stream), "set solution");
-    if (h->dg) h->modal.Limit(h->modal.u0.ptr, h->stream);  // spatial/with_limiter.hpp:56-59
+    if (h->dg) h->modal.SolutionChanged(h->stream);
     Check(hipStreamSynchronize(h->stream), "set solution sync");
   });
 }

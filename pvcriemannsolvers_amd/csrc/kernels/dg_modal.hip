@@ -7,11 +7,15 @@
 //            the global weight -> LDS;  face task: own trace, neighbour trace
 //            (from the neighbour's Taylor coefficients and Jacobian),
 //            Riemann or boundary flux times -/+ the weight -> LDS.
+//   phase 0  the cells' Taylor coefficients, Gram-Schmidt factors and
+//            geometry, and each neighbour's coefficients and Jacobian, are
+//            staged in LDS (one cooperative load per face, not per point);
 //   phase 2  one thread per (cell, component, Taylor monomial) sums the
 //            moments Rt over the cell's tasks;
 //   phase 3  one thread per (cell, basis function, component): R = Rt S^T,
 //            the SSP-RK3 combination (temporal/rk.hpp:31-49 operation order)
-//            and the store.
+//            and the store;
+//   phase 4  the new state's Taylor coefficients for the next pass.
 // An interior face is evaluated by both of its cells from identical inputs
 // (the same table entries, Jacobians and coefficients through the same
 // fixed-order FMA chains), so the two sides see bitwise the same flux and
@@ -67,42 +71,72 @@ __device__ __forceinline__ State EvalTaylor(const double *ct, const double *y) {
   return u;
 }
 
-// Exponents (a, b, c) of Taylor monomial j (basis/taylor.hpp:204-208 order).
-__device__ __forceinline__ void Exponents(int j, int *a, int *b, int *c) {
-  *a = (j == 1 || j == 5 || j == 6) ? 1 : (j == 4 ? 2 : 0);
-  *b = (j == 2 || j == 5 || j == 8) ? 1 : (j == 7 ? 2 : 0);
-  *c = (j == 3 || j == 6 || j == 8) ? 1 : (j == 9 ? 2 : 0);
+// Taylor monomial j of degree <= 2 (basis/taylor.hpp:204-208) as the
+// product of two factors y_a y_b (index 0 stands for the constant 1), and its
+// gradient.  Branch-free: lanes of one wavefront hold different j.
+struct MonomialCode {
+  int a, b;  // factor indices into (1, x, y, z)
+};
+__device__ __forceinline__ MonomialCode MonomialOf(int j) {
+  MonomialCode m;
+  m.a = j == 0 ? 0 : (j <= 3 ? j : (j <= 6 ? 1 : (j <= 8 ? 2 : 3)));
+  m.b = j <= 3 ? 0 : (j == 4 ? 1 : (j == 5 || j == 7 ? 2 : 3));
+  return m;
 }
-
-__device__ __forceinline__ double Power(double x, int e) {  // e in {-1, 0, 1, 2}; x^-1 is never used
-  return e == 0 ? 1.0 : (e == 1 ? x : (e == 2 ? x * x : 0.0));
+__device__ __forceinline__ double Pick(int i, const double *y) {
+  return i == 1 ? y[0] : (i == 2 ? y[1] : (i == 3 ? y[2] : 1.0));
+}
+// T_j(y) and grad T_j(y): d(y_a y_b)/d y_d = [a == d] y_b + [b == d] y_a.
+__device__ __forceinline__ void Monomial(MonomialCode m, const double *y, double *t, double *g) {
+  const double fa = Pick(m.a, y), fb = Pick(m.b, y);
+  *t = fa * fb;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) g[d] = (m.a == d + 1 ? fb : 0.0) + (m.b == d + 1 ? fa : 0.0);
 }
 
 template <bool kHex, int P, int kRiemann>
-__global__ __launch_bounds__(256) void DgModalStageKernel(DgModalArgs a) {
+__global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
   using S = DgShape<kHex, P>;
-  constexpr int N = S::kN, NQ = S::kNQ, NFP = S::kNFP, T = S::kT, G = S::kG;
-  __shared__ double s_ct[G * 5 * N];
-  __shared__ double s_task[G * T * kSlot];
-  __shared__ double s_rt[G * 5 * N];
+  constexpr int N = S::kN, NQ = S::kNQ, NF = S::kNF, NFP = S::kNFP, T = S::kT, G = S::kG, PK = S::kPacked;
+  constexpr int NB = 5 * N + 10;  // neighbour record: Taylor coefficients, J rows, pad
+  constexpr int GS = kDgGeoStride;
+  __shared__ double s_ct[G * 5 * N];       // own Taylor coefficients
+  __shared__ double s_s[G * PK];           // own Gram-Schmidt factors
+  __shared__ double s_geo[G * GS];         // own geometry
+  __shared__ double s_nbr[G * NF * NB];    // neighbours' coefficients and J; then the moments Rt
+  __shared__ double s_vol[G * NQ * kSlot]; // volume tasks: flux x weight (15), offset (3)
+  __shared__ double s_face[G * NF * NFP * 9];  // face tasks: -/+ w Fhat (5), offset (3), pad; then new coefficients
+  static_assert(G * 5 * N <= G * NF * NB && G * 5 * N <= G * NF * NFP * 9, "LDS aliasing");
+  double *s_rt = s_nbr;
+  double *s_new = s_face;
 
   const int tid = threadIdx.x;
   const int64_t c0 = (int64_t)blockIdx.x * G;
   if (c0 >= a.n_cells) return;
   const int nc = (int)min<int64_t>(G, a.n_cells - c0);
 
+  // ---- phase 0: stage the cells' and their neighbours' data in LDS
   for (int w = tid; w < nc * 5 * N; w += 256) s_ct[w] = a.ct[c0 * 5 * N + w];
+  for (int w = tid; w < nc * PK; w += 256) s_s[w] = a.basis[c0 * PK + w];
+  for (int w = tid; w < nc * GS; w += 256) s_geo[w] = a.geo[c0 * GS + w];
+  for (int w = tid; w < nc * NF * NB; w += 256) {
+    const int gl = w / NB, e = w - (w / NB) * NB;
+    const int g = gl / NF, lf = gl - (gl / NF) * NF;
+    const int32_t nb = a.nbr[(c0 + g) * 6 + lf];
+    if (nb < 0 || e >= 5 * N + 9) continue;
+    s_nbr[w] = e < 5 * N ? a.ct[(int64_t)nb * 5 * N + e] : a.geo[(int64_t)nb * GS + (e - 5 * N)];
+  }
   __syncthreads();
 
   // ---- phase 1: point tasks
   const int g = tid / T, r = tid - (tid / T) * T;
   if (g < nc) {
     const int64_t cid = c0 + g;
-    const double *geo = a.geo + cid * kDgGeoStride;
+    const double *geo = s_geo + g * GS;
     const double *own_ct = s_ct + g * 5 * N;
-    double *slot = s_task + (g * T + r) * kSlot;
     const DgTables *tb = a.tables;
     if (r < NQ) {
+      double *slot = s_vol + (g * NQ + r) * kSlot;
       double y[3];
       Offset(geo, tb->vol_eta[r], y);
       const State u = EvalTaylor<P>(own_ct, y);
@@ -117,6 +151,7 @@ __global__ __launch_bounds__(256) void DgModalStageKernel(DgModalArgs a) {
       for (int d = 0; d < 3; ++d) slot[15 + d] = y[d];
     } else {
       const int lf = (r - NQ) / NFP, q = (r - NQ) - ((r - NQ) / NFP) * NFP;
+      double *slot = s_face + (g * NF * NFP + (r - NQ)) * 9;
       const int32_t nb = a.nbr[cid * 6 + lf];
       const int32_t info = a.finfo[cid * 6 + lf];
       const bool holder = (info >> 7) & 1;
@@ -136,9 +171,10 @@ __global__ __launch_bounds__(256) void DgModalStageKernel(DgModalArgs a) {
       double sgn = -1.0;
       if (nb >= 0) {
         const int nf = info & 7;
+        const double *rec = s_nbr + (g * NF + lf) * NB;
         double yn[3];
-        Offset(a.geo + (int64_t)nb * kDgGeoStride, tb->face_eta[nf][holder ? orient : 0][q], yn);
-        const State other = EvalTaylor<P>(a.ct + (int64_t)nb * 5 * N, yn);
+        Offset(rec + 5 * N, tb->face_eta[nf][holder ? orient : 0][q], yn);
+        const State other = EvalTaylor<P>(rec, yn);
         flux = UpwindFlux<kRiemann>(frame, holder ? own : other, holder ? other : own);
         sgn = holder ? -1.0 : 1.0;
       } else {
@@ -161,32 +197,34 @@ __global__ __launch_bounds__(256) void DgModalStageKernel(DgModalArgs a) {
 #pragma unroll
       for (int k = 0; k < 5; ++k) slot[k] = flux.v[k] * (sgn * w);
 #pragma unroll
-      for (int d = 0; d < 3; ++d) slot[15 + d] = y[d];
+      for (int d = 0; d < 3; ++d) slot[5 + d] = y[d];
     }
   }
   __syncthreads();
 
-  // ---- phase 2: Taylor moments Rt[k][j]
+  // ---- phase 2: Taylor moments Rt[k][j], items ordered cell-major so that
+  // the lanes of a wavefront read the same task records (LDS broadcast)
   for (int w = tid; w < nc * 5 * N; w += 256) {
     const int gc = w / (5 * N), rem = w - gc * 5 * N, j = rem / 5, k = rem - (rem / 5) * 5;
-    int ea, eb, ec;
-    Exponents(j, &ea, &eb, &ec);
-    const double *base = s_task + gc * T * kSlot;
+    const MonomialCode m = MonomialOf(j);
+    const double *vb = s_vol + gc * NQ * kSlot;
+    const double *fb = s_face + gc * NF * NFP * 9;
     double acc = 0.0;
+#pragma unroll 2
     for (int q = 0; q < NQ; ++q) {
-      const double *sl = base + q * kSlot;
-      const double x = sl[15], yv = sl[16], z = sl[17];
-      const double px = Power(x, ea), py = Power(yv, eb), pz = Power(z, ec);
-      const double gx = ea ? ea * Power(x, ea - 1) * py * pz : 0.0;
-      const double gy = eb ? eb * px * Power(yv, eb - 1) * pz : 0.0;
-      const double gz = ec ? ec * px * py * Power(z, ec - 1) : 0.0;
-      acc = __fma_rn(sl[3 * k], gx, acc);
-      acc = __fma_rn(sl[3 * k + 1], gy, acc);
-      acc = __fma_rn(sl[3 * k + 2], gz, acc);
+      const double *sl = vb + q * kSlot;
+      double t, gr[3];
+      Monomial(m, sl + 15, &t, gr);
+      acc = __fma_rn(sl[3 * k], gr[0], acc);
+      acc = __fma_rn(sl[3 * k + 1], gr[1], acc);
+      acc = __fma_rn(sl[3 * k + 2], gr[2], acc);
     }
-    for (int q = NQ; q < T; ++q) {
-      const double *sl = base + q * kSlot;
-      acc = __fma_rn(sl[k], Power(sl[15], ea) * Power(sl[16], eb) * Power(sl[17], ec), acc);
+#pragma unroll 2
+    for (int q = 0; q < NF * NFP; ++q) {
+      const double *sl = fb + q * 9;
+      double t, gr[3];
+      Monomial(m, sl + 5, &t, gr);
+      acc = __fma_rn(sl[k], t, acc);
     }
     s_rt[w] = acc;
   }
@@ -195,12 +233,11 @@ __global__ __launch_bounds__(256) void DgModalStageKernel(DgModalArgs a) {
   // ---- phase 3: R = Rt S^T, RK stage, store
   for (int w = tid; w < nc * 5 * N; w += 256) {
     const int gc = w / (5 * N), rem = w - gc * 5 * N, i = rem / 5, k = rem - (rem / 5) * 5;
-    const int64_t cid = c0 + gc;
-    const double *srow = a.basis + cid * S::kPacked + i * (i + 1) / 2;
+    const double *srow = s_s + gc * PK + i * (i + 1) / 2;
     const double *rt = s_rt + gc * 5 * N;
     double res = 0.0;
     for (int j = 0; j <= i; ++j) res = __fma_rn(rt[5 * j + k], srow[j], res);
-    const int64_t idx = cid * 5 * N + rem;
+    const int64_t idx = c0 * 5 * N + w;
     double out;
     if (a.stage == 0) {
       out = res;
@@ -215,6 +252,19 @@ __global__ __launch_bounds__(256) void DgModalStageKernel(DgModalArgs a) {
       }
     }
     a.out[idx] = out;
+    s_new[w] = out;
+  }
+  if (!a.ct_out) return;
+  __syncthreads();
+
+  // ---- phase 4: Taylor coefficients of the new state for the next pass
+  for (int w = tid; w < nc * 5 * N; w += 256) {
+    const int gc = w / (5 * N), rem = w - gc * 5 * N, j = rem / 5, k = rem - (rem / 5) * 5;
+    const double *sn = s_new + gc * 5 * N;
+    const double *sc = s_s + gc * PK;
+    double acc = 0.0;
+    for (int i = j; i < N; ++i) acc = __fma_rn(sn[5 * i + k], sc[i * (i + 1) / 2 + j], acc);
+    a.ct_out[c0 * 5 * N + w] = acc;
   }
 }
 
@@ -416,14 +466,14 @@ __global__ __launch_bounds__(256) void DgWenoProjectKernel(DgLimiterArgs a) {
 }
 
 // weno.hpp:157-194 (Lazy::Reconstruct): nonlinear weights and the weighted
-// sum, one thread per (troubled cell, basis function, component); written
-// over the state only after every projection has been read.
+// sum, one thread per (troubled cell, component); written over the state,
+// with its Taylor coefficients, only after every projection has been read.
 template <int P>
 __global__ __launch_bounds__(256) void DgWenoCombineKernel(DgLimiterArgs a) {
-  constexpr int N = P == 1 ? 4 : 10;
+  constexpr int N = P == 1 ? 4 : 10, PACKED = N * (N + 1) / 2;
   const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int64_t c = w / (5 * N);
-  const int rem = (int)(w - c * 5 * N), k = rem % 5;
+  const int64_t c = w / 5;
+  const int k = (int)(w - c * 5);
   if (c >= a.n_cells || !a.flags[c]) return;
   const int na = a.n_adj[c];
   double wt[kLimiterSlots];
@@ -438,9 +488,22 @@ __global__ __launch_bounds__(256) void DgWenoCombineKernel(DgLimiterArgs a) {
   double sum = 0.0;
   for (int i = 0; i <= na; ++i) sum += wt[i];
   const double *pr = a.proj + c * kLimiterSlots * 5 * N;
-  double v = pr[na * 5 * N + rem] * (wt[na] / sum);
-  for (int i = 0; i < na; ++i) v += pr[i * 5 * N + rem] * (wt[i] / sum);
-  a.u[c * 5 * N + rem] = v;
+  double fresh[10];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double v = pr[na * 5 * N + 5 * i + k] * (wt[na] / sum);
+    for (int s = 0; s < na; ++s) v += pr[s * 5 * N + 5 * i + k] * (wt[s] / sum);
+    fresh[i] = v;
+    a.u[c * 5 * N + 5 * i + k] = v;
+  }
+  const double *sc = a.basis + c * PACKED;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    double acc = 0.0;
+#pragma unroll
+    for (int i = j; i < N; ++i) acc = __fma_rn(fresh[i], sc[i * (i + 1) / 2 + j], acc);
+    a.ct[c * 5 * N + 5 * j + k] = acc;
+  }
 }
 
 template <bool kHex, int P, int R>
@@ -496,7 +559,8 @@ hipError_t LaunchDgLimiter(int cell_type, int degree, const DgLimiterArgs &a, hi
   const int n = degree == 1 ? 4 : 10;
   const int64_t b_flags = (a.n_cells + 255) / 256;
   const int64_t b_proj = (a.n_cells * kLimiterSlots + 255) / 256;
-  const int64_t b_comb = (a.n_cells * 5 * n + 255) / 256;
+  const int64_t b_comb = (a.n_cells * 5 + 255) / 256;
+  (void)n;
   if (b_comb > 0x7fffffff || b_proj > 0x7fffffff) return hipErrorInvalidValue;
   if (degree == 1) {
     hipLaunchKernelGGL(DgWenoFlagsKernel<1>, dim3((unsigned)b_flags), dim3(256), 0, stream, a);

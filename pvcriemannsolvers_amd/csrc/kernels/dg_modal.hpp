@@ -40,6 +40,7 @@ struct DgModalArgs {
   const double *u0;       // U0 for RK stages 2 and 3
   double *out;            // residual (stage 0) or next state
   const double *ct;       // Taylor coefficients of `us`, [cell][N][5]
+  double *ct_out;         // Taylor coefficients of `out`, or null
   const double *basis;    // [cell][N(N+1)/2] S, packed lower triangle by rows
   const double *geo;      // [cell][kDgGeoStride]
   const int32_t *nbr;     // [cell][6]: neighbour cell, or -(1 + boundary tag)
@@ -57,13 +58,13 @@ struct DgModalArgs {
 hipError_t LaunchDgTaylorCoefficients(int degree, const double *us, const double *basis, int64_t n_cells,
                                       double *ct, hipStream_t stream);
 // One residual / RK-stage pass (reads a.ct, which must hold the Taylor
-// coefficients of a.us).
+// coefficients of a.us, and writes those of a.out to a.ct_out if set).
 hipError_t LaunchDgModalStage(int cell_type, int degree, int riemann, const DgModalArgs &a, hipStream_t stream);
 // WENO limiter (limiter/weno.hpp, limiter/reconstruct.hpp) over a state
 // column `u` whose Taylor coefficients are in `ct`.
 struct DgLimiterArgs {
   double *u;              // state, reconstructed in place for troubled cells
-  const double *ct;       // Taylor coefficients of u
+  double *ct;             // Taylor coefficients of u, updated for troubled cells
   const double *basis;    // as DgModalArgs
   const double *geo;
   const int32_t *adj;     // [cell][6] neighbours in interior-face order (Cell::adj_cells_)
