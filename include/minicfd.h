@@ -76,10 +76,15 @@ extern "C" {
  *   cells that the troubled-cell indicator weno::Eigen::IsNotSmooth
  *   (weno.hpp:230-258) flags.
  * WENO_ALL: weno::Lazy on every cell (Lazy::IsNotSmooth is always true; the
- *   configuration of demo/euler/sourceless.hpp:80-81). */
+ *   configuration of demo/euler/sourceless.hpp:80-81).
+ * WENO_EIGEN: weno::Eigen reconstruction (weno.hpp:259-348), i.e. WENO in
+ *   the characteristic variables of each adjacent face, of the flagged cells.
+ * WENO_EIGEN_ALL: the same on every cell. */
 #define MCFD_LIMITER_NONE 0
 #define MCFD_LIMITER_WENO_FLAGGED 1
 #define MCFD_LIMITER_WENO_ALL 2
+#define MCFD_LIMITER_WENO_EIGEN 3
+#define MCFD_LIMITER_WENO_EIGEN_ALL 4
 
 /* ---- boundary conditions, spatial/fem.hpp:196-222 (constant states) */
 #define MCFD_BC_SMART 0               /* SetSmartBoundary(name, f) */

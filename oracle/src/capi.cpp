@@ -323,6 +323,17 @@ int64_t oracle_dg_limit(void *h, double *u, uint8_t *flags) {
   std::copy(p->scheme->flags.begin(), p->scheme->flags.end(), flags);
   return n;
 }
+// riemann/euler/eigen.hpp EigenMatrices of a conservative state in a frame
+// (rows a, b, c); L and R row-major.
+void oracle_eigen_matrices(double const *cv, double const *frame9, double *l, double *r) {
+  Val5 c;
+  std::memcpy(c.data(), cv, 40);
+  Frame f;
+  for (int d = 0; d < 3; ++d) { f.a[d] = frame9[d]; f.b[d] = frame9[3 + d]; f.c[d] = frame9[6 + d]; }
+  double lm[5][5], rm[5][5];
+  DgScheme::EigenMatrices(c, f, lm, rm);
+  for (int i = 0; i < 5; ++i) for (int j = 0; j < 5; ++j) { l[5 * i + j] = lm[i][j]; r[5 * i + j] = rm[i][j]; }
+}
 // Flags of the last limiter application (inside rk3 too).
 void oracle_dg_last_flags(void *h, uint8_t *flags) {
   auto *p = static_cast<DgHandle *>(h);

@@ -108,9 +108,12 @@ struct DgScheme {
   std::vector<ModalBasis> basis;
   // Cell::adj_cells_ (mesh/part.hpp:962-963): neighbours in interior-face order.
   std::vector<std::vector<int64_t>> adj;
+  std::vector<std::vector<int64_t>> adj_faces;  // Cell::adj_faces_ (part.hpp:971-972)
   // Limiter applied at every SetSolutionColumn (spatial/with_limiter.hpp:56-59):
   // 0 none; 1 weno::Lazy reconstruction of the cells weno::Eigen::IsNotSmooth
-  // flags; 2 weno::Lazy on every cell (Lazy::IsNotSmooth is always true).
+  // flags; 2 weno::Lazy on every cell (Lazy::IsNotSmooth is always true);
+  // 3 weno::Eigen (characteristic-wise) reconstruction of the flagged cells;
+  // 4 weno::Eigen reconstruction of every cell.
   int limiter = 0;
   double weno_w0 = 0.001, weno_eps = 1e-6;
   std::vector<uint8_t> flags;  // of the last limiter application
@@ -125,9 +128,12 @@ struct DgScheme {
       basis[c].Build(p, cell.center, cell.qp, cell.qw);
     }
     adj.resize(mesh->cells.size());
+    adj_faces.resize(mesh->cells.size());
     for (int64_t f = 0; f < mesh->n_interior; ++f) {
       adj[mesh->faces[f].holder].push_back(mesh->faces[f].sharer);
       adj[mesh->faces[f].sharer].push_back(mesh->faces[f].holder);
+      adj_faces[mesh->faces[f].holder].push_back(f);
+      adj_faces[mesh->faces[f].sharer].push_back(f);
     }
   }
   size_t Stride() const { return size_t(5) * n; }
@@ -308,8 +314,9 @@ struct DgScheme {
     }
     return beta;
   }
-  // weno.hpp:121-195 (weno::Lazy::Borrow + Reconstruct) of cell c from u.
-  void LazyReconstruct(int64_t c, double const *u, double *out) const {
+  // weno.hpp:121-155 (Borrow): neighbour expansions projected onto cell c's
+  // basis with their averages moved onto c's, then c's own expansion last.
+  std::vector<std::vector<double>> Borrow(int64_t c, double const *u) const {
     auto const &cell = mesh->cells[c];
     auto const &b = basis[c];
     const int n_adj = (int)adj[c].size();
@@ -336,6 +343,12 @@ struct DgScheme {
       }
     }
     std::copy(u + c * Stride(), u + (c + 1) * Stride(), proj[n_adj].begin());
+    return proj;
+  }
+  // Nonlinear weights (weno.hpp:160-177): w0 for neighbours, 1 - n w0 for the
+  // cell itself, divided by (beta + eps)^2 and normalised.
+  std::vector<Val5> Weights(int64_t c, std::vector<std::vector<double>> const &proj) const {
+    const int n_adj = (int)proj.size() - 1;
     std::vector<Val5> weights(n_adj + 1);
     for (int ia = 0; ia <= n_adj; ++ia)
       for (int k = 0; k < 5; ++k) weights[ia][k] = weno_w0;
@@ -351,12 +364,94 @@ struct DgScheme {
     Val5 sum{ 0, 0, 0, 0, 0 };
     for (auto const &w : weights) for (int k = 0; k < 5; ++k) sum[k] += w[k];
     for (auto &w : weights) for (int k = 0; k < 5; ++k) w[k] /= sum[k];
+    return weights;
+  }
+  // new = w_self proj_self + sum_i w_i proj_i (weno.hpp:178-194).
+  void Blend(std::vector<std::vector<double>> const &proj, std::vector<Val5> const &w, double *out) const {
+    const int n_adj = (int)proj.size() - 1;
     for (int i = 0; i < n; ++i)
       for (int k = 0; k < 5; ++k) {
-        double v = proj[n_adj][5 * i + k] * weights[n_adj][k];
-        for (int ia = 0; ia < n_adj; ++ia) v += proj[ia][5 * i + k] * weights[ia][k];
+        double v = proj[n_adj][5 * i + k] * w[n_adj][k];
+        for (int ia = 0; ia < n_adj; ++ia) v += proj[ia][5 * i + k] * w[ia][k];
         out[5 * i + k] = v;
       }
+  }
+  // weno.hpp:121-195 (weno::Lazy::Borrow + Reconstruct) of cell c from u.
+  void LazyReconstruct(int64_t c, double const *u, double *out) const {
+    auto proj = Borrow(c, u);
+    Blend(proj, Weights(c, proj), out);
+  }
+  // riemann/euler/eigen.hpp:26-56 (EigenMatrices) of the state `cv` in the
+  // frame (nu, mu, pi); L and R row-major 5 x 5.
+  static void EigenMatrices(Val5 const &cv, Frame const &f, double l[5][5], double r[5][5]) {
+    Val5 pv = ConservativeToPrimitive(cv);
+    const double ux = pv[1], uy = pv[2], uz = pv[3];
+    const Vec3 &nu = f.a, &mu = f.b, &pi = f.c;
+    const double u_nu = ux * nu[0] + uy * nu[1] + uz * nu[2];
+    const double u_mu = ux * mu[0] + uy * mu[1] + uz * mu[2];
+    const double u_pi = ux * pi[0] + uy * pi[1] + uz * pi[2];
+    const double a = std::sqrt(1.4 * pv[4] / pv[0]);
+    const double ek = (ux * ux + uy * uy + uz * uz) / 2;
+    const double e0 = a * a * (1 / (1.4 - 1));
+    const double h0 = e0 + ek;
+    const double rc[5][5] = {  // columns of R
+      { 1, ux - a * nu[0], uy - a * nu[1], uz - a * nu[2], h0 - a * u_nu },
+      { 1, ux, uy, uz, ek },
+      { 0, mu[0], mu[1], mu[2], u_mu },
+      { 0, pi[0], pi[1], pi[2], u_pi },
+      { 1, ux + a * nu[0], uy + a * nu[1], uz + a * nu[2], h0 + a * u_nu },
+    };
+    for (int i = 0; i < 5; ++i) for (int j = 0; j < 5; ++j) r[i][j] = rc[j][i];
+    const double b1 = 1 / e0, b2 = b1 * ek;
+    const double lc[5][5] = {  // columns of L
+      { (b2 + u_nu / a) / 2, 1 - b2, -u_mu, -u_pi, (b2 - u_nu / a) / 2 },
+      { -(b1 * ux + nu[0] / a) / 2, b1 * ux, mu[0], pi[0], -(b1 * ux - nu[0] / a) / 2 },
+      { -(b1 * uy + nu[1] / a) / 2, b1 * uy, mu[1], pi[1], -(b1 * uy - nu[1] / a) / 2 },
+      { -(b1 * uz + nu[2] / a) / 2, b1 * uz, mu[2], pi[2], -(b1 * uz - nu[2] / a) / 2 },
+      { b1 / 2, -b1, 0, 0, b1 / 2 },
+    };
+    for (int i = 0; i < 5; ++i) for (int j = 0; j < 5; ++j) l[i][j] = lc[j][i];
+  }
+  // weno.hpp:292-348 (weno::Eigen::ReconstructOnFace + Reconstruct): per
+  // adjacent face, WENO in the characteristic variables of the cell average
+  // in that face's frame, mapped back and averaged with the neighbours'
+  // volumes as weights.
+  void EigenReconstruct(int64_t c, double const *u, double *out) const {
+    auto proj = Borrow(c, u);
+    const Val5 big_u = Average(c, u + c * Stride());
+    std::vector<double> acc(Stride(), 0.0), tmp(Stride());
+    double total_volume = 0;
+    for (size_t i_face = 0; i_face < adj_faces[c].size(); ++i_face) {
+      auto const &face = mesh->faces[adj_faces[c][i_face]];
+      double l[5][5], r[5][5];
+      EigenMatrices(big_u, face.frames[0], l, r);
+      auto rotated = proj;
+      for (auto &pr : rotated) {  // Expansion::LeftMultiply(L)
+        for (int i = 0; i < n; ++i) {
+          double col[5];
+          for (int k = 0; k < 5; ++k) col[k] = pr[5 * i + k];
+          for (int k = 0; k < 5; ++k) {
+            double v = 0;
+            for (int m = 0; m < 5; ++m) v += l[k][m] * col[m];
+            pr[5 * i + k] = v;
+          }
+        }
+      }
+      Blend(rotated, Weights(c, rotated), tmp.data());
+      const int64_t other = face.holder == c ? face.sharer : face.holder;
+      const double adj_volume = mesh->cells[other].volume;
+      for (int i = 0; i < n; ++i) {
+        double col[5];
+        for (int k = 0; k < 5; ++k) col[k] = tmp[5 * i + k];
+        for (int k = 0; k < 5; ++k) {
+          double v = 0;
+          for (int m = 0; m < 5; ++m) v += r[k][m] * col[m];
+          acc[5 * i + k] += v * adj_volume;
+        }
+      }
+      total_volume += adj_volume;
+    }
+    for (size_t i = 0; i < Stride(); ++i) out[i] = acc[i] / total_volume;
   }
   // limiter/reconstruct.hpp:22-47: flag all cells, reconstruct the troubled
   // ones from the old state, then write.  Returns the number of troubled cells.
@@ -366,11 +461,14 @@ struct DgScheme {
     if (limiter == 0) return 0;
     std::vector<int64_t> troubled;
     for (size_t c = 0; c < nc; ++c) {
-      flags[c] = limiter == 2 ? 1 : (IsNotSmooth(c, u) ? 1 : 0);
+      flags[c] = (limiter == 2 || limiter == 4) ? 1 : (IsNotSmooth(c, u) ? 1 : 0);
       if (flags[c]) troubled.push_back(c);
     }
     std::vector<double> fresh(troubled.size() * Stride());
-    for (size_t t = 0; t < troubled.size(); ++t) LazyReconstruct(troubled[t], u, fresh.data() + t * Stride());
+    for (size_t t = 0; t < troubled.size(); ++t) {
+      if (limiter >= 3) EigenReconstruct(troubled[t], u, fresh.data() + t * Stride());
+      else LazyReconstruct(troubled[t], u, fresh.data() + t * Stride());
+    }
     for (size_t t = 0; t < troubled.size(); ++t)
       std::copy(fresh.begin() + t * Stride(), fresh.begin() + (t + 1) * Stride(), u + troubled[t] * Stride());
     return (int64_t)troubled.size();

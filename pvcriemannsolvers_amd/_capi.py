@@ -20,7 +20,7 @@ NODES_LEGENDRE, NODES_LOBATTO = 0, 1
 RIEMANN_EXACT, RIEMANN_HLLC, RIEMANN_ROE = 0, 1, 2
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 FLAG_HOST_ONLY = 1
-LIMITER_NONE, LIMITER_WENO_FLAGGED, LIMITER_WENO_ALL = 0, 1, 2
+LIMITER_NONE, LIMITER_WENO_FLAGGED, LIMITER_WENO_ALL, LIMITER_WENO_EIGEN, LIMITER_WENO_EIGEN_ALL = 0, 1, 2, 3, 4
 BC_SMART, BC_INVISCID_WALL, BC_SUPERSONIC_OUTLET, BC_SUPERSONIC_INLET = 0, 1, 2, 3
 
 _dp = ctypes.POINTER(ctypes.c_double)

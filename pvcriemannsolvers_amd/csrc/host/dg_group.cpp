@@ -143,7 +143,7 @@ void DgGroup::Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &scheme,
   cell_type = mesh.cell_type;
   degree = scheme.degree;
   riemann = scheme.riemann;
-  if (scheme.limiter < MCFD_LIMITER_NONE || scheme.limiter > MCFD_LIMITER_WENO_ALL)
+  if (scheme.limiter < MCFD_LIMITER_NONE || scheme.limiter > MCFD_LIMITER_WENO_EIGEN_ALL)
     throw std::invalid_argument("unknown limiter");
   limiter = scheme.limiter;
   weno_w0 = scheme.weno_w0;
@@ -367,12 +367,15 @@ void DgGroup::Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &scheme,
 
   // ---- limiter neighbours (mesh/part.hpp:962-963): interior faces in order
   h_adj.assign(n_cells * 6, -1);
+  h_adj_face.assign(n_cells * 6, -1);
   h_nadj.assign(n_cells, 0);
   for (int64_t f = 0; f < n_interior; ++f) {
     const auto &face = faces[f];
     if (limiter && (shift[f][0] != 0 || shift[f][1] != 0 || shift[f][2] != 0))
       throw std::invalid_argument("the WENO limiter is not defined across periodic faces");
+    h_adj_face[face.holder * 6 + h_nadj[face.holder]] = (int32_t)f;
     h_adj[face.holder * 6 + h_nadj[face.holder]++] = (int32_t)face.sharer;
+    h_adj_face[face.sharer * 6 + h_nadj[face.sharer]] = (int32_t)f;
     h_adj[face.sharer * 6 + h_nadj[face.sharer]++] = (int32_t)face.holder;
   }
 }
@@ -394,11 +397,13 @@ void DgGroup::Upload(hipStream_t stream) {
   tables_d.Upload(h_tables, stream);
   if (limiter) {
     adj.Upload(h_adj, stream);
+    adj_face.Upload(h_adj_face, stream);
     nadj.Upload(h_nadj, stream);
     flags.Allocate(n_cells);
     Check(hipMemsetAsync(flags.ptr, 0, n_cells, stream), "memset");
     proj.Allocate(size_t(n_cells) * 7 * 5 * n);
     beta.Allocate(size_t(n_cells) * 7 * 5);
+    if (limiter >= MCFD_LIMITER_WENO_EIGEN) eig.Allocate(size_t(n_cells) * 6 * 5 * n);
   }
 }
 
@@ -406,8 +411,8 @@ void DgGroup::Limit(double *u, double *ct, hipStream_t stream) {
   if (!limiter) return;
   DgLimiterArgs a;
   a.u = u; a.ct = ct; a.basis = basis.ptr; a.geo = geo.ptr;
-  a.adj = adj.ptr; a.n_adj = nadj.ptr; a.tables = tables_d.ptr;
-  a.flags = flags.ptr; a.proj = proj.ptr; a.beta = beta.ptr;
+  a.adj = adj.ptr; a.adj_face = adj_face.ptr; a.n_adj = nadj.ptr; a.faces = face_rec.ptr; a.tables = tables_d.ptr;
+  a.flags = flags.ptr; a.proj = proj.ptr; a.beta = beta.ptr; a.eig = eig.ptr;
   a.n_cells = n_cells; a.mode = limiter; a.w0 = weno_w0; a.eps = weno_eps;
   Check(LaunchDgLimiter(cell_type, degree, a, stream), "DG limiter launch");
 }

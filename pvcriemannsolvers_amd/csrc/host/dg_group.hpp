@@ -38,10 +38,10 @@ struct DgGroup {
   // WENO limiter (MCFD_LIMITER_*), applied to every state that is set
   int limiter = 0;
   double weno_w0 = 0.001, weno_eps = 1e-6;
-  std::vector<int32_t> h_adj, h_nadj;  // Cell::adj_cells_ in interior-face order, [cell][6]
-  DeviceArray<int32_t> adj, nadj;
+  std::vector<int32_t> h_adj, h_adj_face, h_nadj;  // Cell::adj_cells_/adj_faces_ in face order, [cell][6]
+  DeviceArray<int32_t> adj, adj_face, nadj;
   DeviceArray<uint8_t> flags;
-  DeviceArray<double> proj, beta;
+  DeviceArray<double> proj, beta, eig;
 
   void Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &scheme,
              const std::vector<int32_t> &bc_kinds, const std::vector<double> &bc_states);

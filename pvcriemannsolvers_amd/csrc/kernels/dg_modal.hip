@@ -324,7 +324,7 @@ __global__ __launch_bounds__(256) void DgWenoFlagsKernel(DgLimiterArgs a) {
   constexpr int N = P == 1 ? 4 : 10, PACKED = N * (N + 1) / 2;
   const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (c >= a.n_cells) return;
-  if (a.mode == 2) {
+  if (a.mode == 2 || a.mode == 4) {
     a.flags[c] = 1;
     return;
   }
@@ -352,16 +352,16 @@ __global__ __launch_bounds__(256) void DgWenoFlagsKernel(DgLimiterArgs a) {
   a.flags[c] = rough ? 1 : 0;
 }
 
-// weno.hpp:74-95 (GetSmoothness) of component k of coefficients `pr` ([N][5])
-// on the basis `s` of a cell with geometry `gc`.
+// weno.hpp:74-95 (GetSmoothness) of the scalar expansion c[stride * i]
+// (i < N) on the basis `s` of a cell with geometry `gc`.
 template <bool kHex, int P>
-__device__ double Smoothness(const double *pr, int k, const double *s, const double *gc, const DgTables *tb) {
+__device__ double Smoothness(const double *c, int stride, const double *s, const double *gc, const DgTables *tb) {
   constexpr int N = P == 1 ? 4 : 10, NQ = DgShape<kHex, P>::kNQ;
-  double tc[10];  // Taylor coefficients of component k: (C S)[k][j]
+  double tc[10];  // Taylor coefficients (c S)[j]
 #pragma unroll
   for (int j = 0; j < N; ++j) {
     double acc = 0.0;
-    for (int i = j; i < N; ++i) acc = __fma_rn(pr[5 * i + k], s[i * (i + 1) / 2 + j], acc);
+    for (int i = j; i < N; ++i) acc = __fma_rn(c[stride * i], s[i * (i + 1) / 2 + j], acc);
     tc[j] = acc;
   }
   double integ[10];
@@ -461,8 +461,9 @@ __global__ __launch_bounds__(256) void DgWenoProjectKernel(DgLimiterArgs a) {
   for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int k = 0; k < 5; ++k) pr[5 * i + k] = acc[i][k];
+  if (a.mode >= 3) return;  // weno::Eigen measures smoothness in characteristic variables
   for (int k = 0; k < 5; ++k)
-    a.beta[(c * kLimiterSlots + slot) * 5 + k] = Smoothness<kHex, P>(pr, k, s, gc, tb);
+    a.beta[(c * kLimiterSlots + slot) * 5 + k] = Smoothness<kHex, P>(pr + k, 5, s, gc, tb);
 }
 
 // weno.hpp:157-194 (Lazy::Reconstruct): nonlinear weights and the weighted
@@ -503,6 +504,160 @@ __global__ __launch_bounds__(256) void DgWenoCombineKernel(DgLimiterArgs a) {
 #pragma unroll
     for (int i = j; i < N; ++i) acc = __fma_rn(fresh[i], sc[i * (i + 1) / 2 + j], acc);
     a.ct[c * 5 * N + 5 * j + k] = acc;
+  }
+}
+
+// riemann/euler/eigen.hpp:26-56: left and right eigenvector matrices of the
+// Euler flux Jacobian at state `cv` in the frame (n, t, b), row-major.
+__device__ void EigenMatrices(const State &cv, const double *n, const double *t, const double *b,
+                              double l[5][5], double r[5][5]) {
+  const State pv = ToPrimitive(cv);
+  const double ux = pv.v[1], uy = pv.v[2], uz = pv.v[3];
+  const double u_nu = ux * n[0] + uy * n[1] + uz * n[2];
+  const double u_mu = ux * t[0] + uy * t[1] + uz * t[2];
+  const double u_pi = ux * b[0] + uy * b[1] + uz * b[2];
+  const double a = sqrt(kGamma * pv.v[4] / pv.v[0]);
+  const double ek = (ux * ux + uy * uy + uz * uz) / 2;
+  const double e0 = a * a * (1 / kGm1);
+  const double h0 = e0 + ek;
+  const double u[3] = { ux, uy, uz };
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    r[1 + d][0] = u[d] - a * n[d];
+    r[1 + d][1] = u[d];
+    r[1 + d][2] = t[d];
+    r[1 + d][3] = b[d];
+    r[1 + d][4] = u[d] + a * n[d];
+  }
+  r[0][0] = 1; r[0][1] = 1; r[0][2] = 0; r[0][3] = 0; r[0][4] = 1;
+  r[4][0] = h0 - a * u_nu; r[4][1] = ek; r[4][2] = u_mu; r[4][3] = u_pi; r[4][4] = h0 + a * u_nu;
+  const double b1 = 1 / e0, b2 = b1 * ek;
+  l[0][0] = (b2 + u_nu / a) / 2; l[1][0] = 1 - b2; l[2][0] = -u_mu; l[3][0] = -u_pi; l[4][0] = (b2 - u_nu / a) / 2;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    l[0][1 + d] = -(b1 * u[d] + n[d] / a) / 2;
+    l[1][1 + d] = b1 * u[d];
+    l[2][1 + d] = t[d];
+    l[3][1 + d] = b[d];
+    l[4][1 + d] = -(b1 * u[d] - n[d] / a) / 2;
+  }
+  l[0][4] = b1 / 2; l[1][4] = -b1; l[2][4] = 0; l[3][4] = 0; l[4][4] = b1 / 2;
+}
+
+__device__ __forceinline__ void FaceFrameOf(const double *fc, double *n, double *t, double *b) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) { n[d] = fc[d]; t[d] = fc[3 + d]; }
+  b[0] = n[1] * t[2] - n[2] * t[1];
+  b[1] = n[2] * t[0] - n[0] * t[2];
+  b[2] = n[0] * t[1] - n[1] * t[0];
+}
+
+// weno.hpp:292-337 (weno::Eigen::ReconstructOnFace): one thread per
+// (troubled cell, adjacent face, characteristic field k).  Every borrowed
+// projection is mapped by row k of L (cell average, face frame), its
+// smoothness weighs it, and the blended field-k coefficients go to a.eig.
+template <bool kHex, int P>
+__global__ __launch_bounds__(256) void DgWenoEigenKernel(DgLimiterArgs a) {
+  constexpr int N = P == 1 ? 4 : 10, PACKED = N * (N + 1) / 2;
+  const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t c = w / 30;
+  const int rem = (int)(w - c * 30), f = rem / 5, k = rem - (rem / 5) * 5;
+  if (c >= a.n_cells || !a.flags[c]) return;
+  const int na = a.n_adj[c];
+  if (f >= na) return;
+  const double *s = a.basis + c * PACKED;
+  const double *gc = a.geo + c * kDgGeoStride;
+  const double *pr = a.proj + c * kLimiterSlots * 5 * N;
+  State avg;  // Projection::average of the cell's own expansion (projection slot na)
+#pragma unroll
+  for (int m = 0; m < 5; ++m) avg.v[m] = pr[na * 5 * N + m] * s[0];
+  double n[3], t[3], b[3];
+  FaceFrameOf(a.faces + (int64_t)a.adj_face[c * 6 + f] * kDgFaceStride, n, t, b);
+  double l[5][5], r[5][5];
+  EigenMatrices(avg, n, t, b, l, r);
+  double wt[kLimiterSlots];
+  for (int i = 0; i <= na; ++i) {
+    double ch[10];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      double v = 0.0;
+#pragma unroll
+      for (int m = 0; m < 5; ++m) v += l[k][m] * pr[i * 5 * N + 5 * j + m];
+      ch[j] = v;
+    }
+    double beta = Smoothness<kHex, P>(ch, 1, s, gc, a.tables);
+    double wi = a.w0;
+    if (i == na) { wi *= -na; wi += 1.0; }
+    beta += a.eps;
+    beta *= beta;
+    wt[i] = wi / beta;
+  }
+  double sum = 0.0;
+  for (int i = 0; i <= na; ++i) sum += wt[i];
+  double *out = a.eig + ((c * 6 + f) * 5 + k) * N;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    double ch_self = 0.0;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) ch_self += l[k][m] * pr[na * 5 * N + 5 * j + m];
+    double v = ch_self * (wt[na] / sum);
+    for (int i = 0; i < na; ++i) {
+      double ch = 0.0;
+#pragma unroll
+      for (int m = 0; m < 5; ++m) ch += l[k][m] * pr[i * 5 * N + 5 * j + m];
+      v += ch * (wt[i] / sum);
+    }
+    out[j] = v;
+  }
+}
+
+// weno.hpp:317-347: back to conservative variables with each face's R, the
+// neighbours' volumes as weights; one thread per (troubled cell, component).
+template <int P>
+__global__ __launch_bounds__(256) void DgWenoEigenCombineKernel(DgLimiterArgs a) {
+  constexpr int N = P == 1 ? 4 : 10, PACKED = N * (N + 1) / 2;
+  const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t c = w / 5;
+  const int m = (int)(w - c * 5);
+  if (c >= a.n_cells || !a.flags[c]) return;
+  const int na = a.n_adj[c];
+  const double *s = a.basis + c * PACKED;
+  const double *pr = a.proj + c * kLimiterSlots * 5 * N;
+  State avg;  // from the own-projection slot: a.u is written below by sibling threads
+#pragma unroll
+  for (int q = 0; q < 5; ++q) avg.v[q] = pr[na * 5 * N + q] * s[0];
+  double acc[10];
+#pragma unroll
+  for (int i = 0; i < N; ++i) acc[i] = 0.0;
+  double total = 0.0;
+  for (int f = 0; f < na; ++f) {
+    double n[3], t[3], b[3];
+    FaceFrameOf(a.faces + (int64_t)a.adj_face[c * 6 + f] * kDgFaceStride, n, t, b);
+    double l[5][5], r[5][5];
+    EigenMatrices(avg, n, t, b, l, r);
+    const double vol = a.geo[(int64_t)a.adj[c * 6 + f] * kDgGeoStride + 14];
+    const double *e = a.eig + (c * 6 + f) * 5 * N;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      double v = 0.0;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) v += r[m][k] * e[k * N + i];
+      acc[i] += v * vol;
+    }
+    total += vol;
+  }
+  double fresh[10];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    fresh[i] = acc[i] / total;
+    a.u[c * 5 * N + 5 * i + m] = fresh[i];
+  }
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    double v = 0.0;
+#pragma unroll
+    for (int i = j; i < N; ++i) v = __fma_rn(fresh[i], s[i * (i + 1) / 2 + j], v);
+    a.ct[c * 5 * N + 5 * j + m] = v;
   }
 }
 
@@ -562,25 +717,34 @@ hipError_t LaunchDgLimiter(int cell_type, int degree, const DgLimiterArgs &a, hi
   const int64_t b_comb = (a.n_cells * 5 + 255) / 256;
   (void)n;
   if (b_comb > 0x7fffffff || b_proj > 0x7fffffff) return hipErrorInvalidValue;
+  const int64_t b_eig = (a.n_cells * 30 + 255) / 256;
+  if (b_eig > 0x7fffffff) return hipErrorInvalidValue;
+  const bool eigen = a.mode >= 3;
+#define MCFD_LIMITER_LAUNCH(P)                                                                          \
+  hipLaunchKernelGGL(DgWenoFlagsKernel<P>, dim3((unsigned)b_flags), dim3(256), 0, stream, a);           \
+  if (hex) {                                                                                            \
+    hipLaunchKernelGGL((DgWenoProjectKernel<true, P>), dim3((unsigned)b_proj), dim3(256), 0, stream, a); \
+  } else {                                                                                              \
+    hipLaunchKernelGGL((DgWenoProjectKernel<false, P>), dim3((unsigned)b_proj), dim3(256), 0, stream, a); \
+  }                                                                                                     \
+  if (!eigen) {                                                                                         \
+    hipLaunchKernelGGL(DgWenoCombineKernel<P>, dim3((unsigned)b_comb), dim3(256), 0, stream, a);        \
+  } else {                                                                                              \
+    if (hex) {                                                                                          \
+      hipLaunchKernelGGL((DgWenoEigenKernel<true, P>), dim3((unsigned)b_eig), dim3(256), 0, stream, a); \
+    } else {                                                                                            \
+      hipLaunchKernelGGL((DgWenoEigenKernel<false, P>), dim3((unsigned)b_eig), dim3(256), 0, stream, a); \
+    }                                                                                                   \
+    hipLaunchKernelGGL(DgWenoEigenCombineKernel<P>, dim3((unsigned)b_comb), dim3(256), 0, stream, a);   \
+  }
   if (degree == 1) {
-    hipLaunchKernelGGL(DgWenoFlagsKernel<1>, dim3((unsigned)b_flags), dim3(256), 0, stream, a);
-    if (hex) {
-      hipLaunchKernelGGL((DgWenoProjectKernel<true, 1>), dim3((unsigned)b_proj), dim3(256), 0, stream, a);
-    } else {
-      hipLaunchKernelGGL((DgWenoProjectKernel<false, 1>), dim3((unsigned)b_proj), dim3(256), 0, stream, a);
-    }
-    hipLaunchKernelGGL(DgWenoCombineKernel<1>, dim3((unsigned)b_comb), dim3(256), 0, stream, a);
+    MCFD_LIMITER_LAUNCH(1)
   } else if (degree == 2) {
-    hipLaunchKernelGGL(DgWenoFlagsKernel<2>, dim3((unsigned)b_flags), dim3(256), 0, stream, a);
-    if (hex) {
-      hipLaunchKernelGGL((DgWenoProjectKernel<true, 2>), dim3((unsigned)b_proj), dim3(256), 0, stream, a);
-    } else {
-      hipLaunchKernelGGL((DgWenoProjectKernel<false, 2>), dim3((unsigned)b_proj), dim3(256), 0, stream, a);
-    }
-    hipLaunchKernelGGL(DgWenoCombineKernel<2>, dim3((unsigned)b_comb), dim3(256), 0, stream, a);
+    MCFD_LIMITER_LAUNCH(2)
   } else {
     return hipErrorInvalidValue;
   }
+#undef MCFD_LIMITER_LAUNCH
   return hipGetLastError();
 }
 

@@ -68,13 +68,16 @@ struct DgLimiterArgs {
   const double *basis;    // as DgModalArgs
   const double *geo;
   const int32_t *adj;     // [cell][6] neighbours in interior-face order (Cell::adj_cells_)
+  const int32_t *adj_face;  // [cell][6] the faces to them (Cell::adj_faces_)
   const int32_t *n_adj;   // [cell]
+  const double *faces;    // face records, as DgModalArgs
   const DgTables *tables;
   uint8_t *flags;         // [cell]
   double *proj;           // scratch [cell][7][N][5]: neighbour projections, then the cell's own
-  double *beta;           // scratch [cell][7][5]: their smoothness indicators
+  double *beta;           // scratch [cell][7][5]: their smoothness indicators (Lazy)
+  double *eig;            // scratch [cell][6][5][N]: blended characteristic fields per face (Eigen)
   int64_t n_cells;
-  int32_t mode;           // 1 flagged by weno::Eigen::IsNotSmooth, 2 all cells
+  int32_t mode;           // MCFD_LIMITER_*: 1, 2 weno::Lazy on flagged / all cells; 3, 4 weno::Eigen
   double w0, eps;
 };
 hipError_t LaunchDgLimiter(int cell_type, int degree, const DgLimiterArgs &a, hipStream_t stream);

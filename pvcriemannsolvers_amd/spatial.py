@@ -43,7 +43,8 @@ class FiniteElement(System):
     _BC = {"smart": C.BC_SMART, "inviscid_wall": C.BC_INVISCID_WALL,
            "supersonic_outlet": C.BC_SUPERSONIC_OUTLET, "supersonic_inlet": C.BC_SUPERSONIC_INLET}
 
-    _LIMITER = {"none": C.LIMITER_NONE, "weno": C.LIMITER_WENO_FLAGGED, "weno_all": C.LIMITER_WENO_ALL}
+    _LIMITER = {"none": C.LIMITER_NONE, "weno": C.LIMITER_WENO_FLAGGED, "weno_all": C.LIMITER_WENO_ALL,
+                "weno_eigen": C.LIMITER_WENO_EIGEN, "weno_eigen_all": C.LIMITER_WENO_EIGEN_ALL}
 
     def __init__(self, mesh: dict, degree: int, scheme: str = "fr", riemann: str = "roe",
                  nodes: str = "legendre", c_next: float = 1.0, bcs=(), precision: str = "fp64",

@@ -125,6 +125,16 @@ def wall_flux(frame, c):
     return out
 
 
+def eigen_matrices(cv, frame):
+    """riemann/euler/eigen.hpp: (L, R) of state cv in the frame rows (n, t, b)."""
+    lib = load()
+    lib.oracle_eigen_matrices.argtypes = [_dp, _dp, _dp, _dp]
+    cv, frame = f64(cv), f64(frame).reshape(9)
+    l, r = np.zeros((5, 5)), np.zeros((5, 5))
+    lib.oracle_eigen_matrices(_ptr(cv, _dp), _ptr(frame, _dp), _ptr(l, _dp), _ptr(r, _dp))
+    return l, r
+
+
 class FrOracle:
     """Nodal FR on hexahedra (spatial/fr/general.hpp restated)."""
 

@@ -168,13 +168,16 @@ def quadrant_state(x, seed=21):
     return cons(rho[q], (vel[q, 0], vel[q, 1], 0.0 * q), p[q])
 
 
-@pytest.mark.parametrize("mode", ["weno", "weno_all"])
+LIMITER_MODES = {"weno": 1, "weno_all": 2, "weno_eigen": 3, "weno_eigen_all": 4}
+
+
+@pytest.mark.parametrize("mode", sorted(LIMITER_MODES))
 @pytest.mark.parametrize("name", sorted(MESHES))
 def test_limiter_flags_and_state_match_oracle(mode, name):
     mesh = MESHES[name]()
     bcs = [("inviscid_wall", None)] * 6
     o = ol.DgOracle(mesh, degree=2, riemann=1, bcs=[(1, [0] * 5)] * 6)
-    o.set_limiter({"weno": 1, "weno_all": 2}[mode])
+    o.set_limiter(LIMITER_MODES[mode])
     g = spatial.FiniteElement(mesh, degree=2, scheme="dg", riemann="hllc", bcs=bcs, limiter=mode)
     x = o.cells()["xyz"]
     vals = quadrant_state(x)
@@ -187,13 +190,14 @@ def test_limiter_flags_and_state_match_oracle(mode, name):
     g.close()
 
 
-def test_limited_rk3_on_tets_matches_oracle():
+@pytest.mark.parametrize("mode", ["weno", "weno_eigen"])
+def test_limited_rk3_on_tets_matches_oracle(mode):
     """4-quadrant problem on jittered tetrahedra, HLLC, WENO on flagged cells, 3 SSP-RK3 steps."""
     mesh = meshgen.box_tet(8, 8, 2, hi=(1.0, 1.0, 0.25), jitter=0.15, seed=5)
     bcs = [("inviscid_wall", None)] * 6
     o = ol.DgOracle(mesh, degree=2, riemann=1, bcs=[(1, [0] * 5)] * 6)
-    o.set_limiter(1)
-    g = spatial.FiniteElement(mesh, degree=2, scheme="dg", riemann="hllc", bcs=bcs, limiter="weno")
+    o.set_limiter(LIMITER_MODES[mode])
+    g = spatial.FiniteElement(mesh, degree=2, scheme="dg", riemann="hllc", bcs=bcs, limiter=mode)
     vals = quadrant_state(g.sample_points())
     g.approximate(vals)
     u, _ = o.limit(o.approximate(vals))

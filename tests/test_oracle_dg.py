@@ -332,3 +332,61 @@ def test_lazy_reconstruction_properties(name):
     v, _ = o.limit(noisy.reshape(-1))
     v = v.reshape(o.n_cells, o.n, 5)
     assert np.abs(v[target] - u[target]).max() < 1e-3 * np.abs(u[target]).max()
+
+
+def test_eigen_matrices_invert_each_other():
+    """test/riemann/euler/eigen.cpp:13-24: |RL - I|, |LR - I| < 1e-15 (scaled here by the state)."""
+    rng = np.random.default_rng(17)
+    for _ in range(16):
+        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        if np.linalg.det(q) < 0:
+            q[:, 2] *= -1
+        cv = cons(rng.uniform(0.5, 2), rng.uniform(-1, 1, 3), rng.uniform(0.5, 2))
+        l, r = ol.eigen_matrices(cv, q)
+        np.testing.assert_allclose(l @ r, np.eye(5), atol=1e-14)
+        np.testing.assert_allclose(r @ l, np.eye(5), atol=1e-14)
+        # R's columns are right eigenvectors of the normal flux Jacobian: A_n R = R diag(u-a, u, u, u, u+a)
+        n = q[0]
+        eps = 1e-7
+        jac = np.zeros((5, 5))
+        for j in range(5):
+            dp, dm = cv.copy(), cv.copy()
+            dp[j] += eps
+            dm[j] -= eps
+            jac[:, j] = (ol.flux_matrix(dp) @ n - ol.flux_matrix(dm) @ n) / (2 * eps)
+        pv = ol.conservative_to_primitive(cv)
+        un, a = pv[1:4] @ n, np.sqrt(GAMMA * pv[4] / pv[0])
+        np.testing.assert_allclose(jac @ r, r @ np.diag([un - a, un, un, un, un + a]), atol=1e-6)
+
+
+@pytest.mark.parametrize("name", sorted(MESHES))
+def test_eigen_reconstruction_properties(name):
+    """weno::Eigen: averages kept, linear data unchanged, a smooth cell among rough neighbours kept."""
+    mesh = MESHES[name]()
+    o = ol.DgOracle(mesh, degree=2, riemann=HLLC, bcs=[(WALL, [0] * 5)] * 6)
+    o.set_limiter(3)
+    c = o.cells()
+    u = o.approximate(rough_state(c["xyz"]))
+    v, flags = o.limit(u)
+    assert flags.any()
+    cu, cv = u.reshape(o.n_cells, o.n, 5), v.reshape(o.n_cells, o.n, 5)
+    np.testing.assert_allclose(cv[:, 0, :], cu[:, 0, :], rtol=1e-12, atol=1e-12 * np.abs(cu[:, 0, :]).max())
+    assert np.array_equal(cv[flags == 0], cu[flags == 0])
+    x = c["xyz"]
+    lin = cons(1 + 0.2 * x[..., 0] - 0.1 * x[..., 1], (0.1, 0.2, 0.0), 1 + 0.3 * x[..., 2])
+    u = o.approximate(lin)
+    v, flags = o.limit(u)
+    assert not flags.any() and np.array_equal(v, u)
+    o.set_limiter(4)  # every cell: the characteristic blend of identical projections is the identity
+    v, flags = o.limit(u)
+    assert flags.all()
+    np.testing.assert_allclose(v, u, rtol=0, atol=1e-12 * np.abs(u).max())
+    # one smooth cell among rough neighbours keeps its own polynomial
+    rng = np.random.default_rng(3)
+    noisy = u.reshape(o.n_cells, o.n, 5).copy()
+    noisy[:, 1:, :] += 0.3 * rng.standard_normal(noisy[:, 1:, :].shape)
+    noisy[0] = u.reshape(o.n_cells, o.n, 5)[0]
+    v, _ = o.limit(noisy.reshape(-1))
+    v = v.reshape(o.n_cells, o.n, 5)
+    ref = u.reshape(o.n_cells, o.n, 5)[0]
+    assert np.abs(v[0] - ref).max() < 1e-3 * np.abs(ref).max()
