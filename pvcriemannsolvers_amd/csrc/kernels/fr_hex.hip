@@ -403,9 +403,16 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
 template <int Q, int Cells, int kRiemann, bool kDiag, bool kMixed>
 __global__ __launch_bounds__(256) void FrHexStageKernel(FrHexArgs a, LineTables lt) {
   __shared__ double lds[Shape<Q, Cells>::kLdsDoubles];
+  // The line tables are indexed by lane-varying node indices: read from the
+  // kernel arguments they would be vector loads on the critical path of every
+  // pass, so they are staged in LDS once (the body's first barrier orders it).
+  __shared__ LineTables s_lt;
+  constexpr int kTableDoubles = sizeof(LineTables) / sizeof(double);
+  if (threadIdx.x < kTableDoubles)
+    reinterpret_cast<double *>(&s_lt)[threadIdx.x] = reinterpret_cast<const double *>(&lt)[threadIdx.x];
   int64_t block = LogicalBlock(a.xcd_remap);
   if (a.tile_order) block = a.tile_order[block];
-  FrHexStageBody<Q, Cells, kRiemann, kDiag, kMixed>(a, lt, block, lds);
+  FrHexStageBody<Q, Cells, kRiemann, kDiag, kMixed>(a, s_lt, block, lds);
 }
 
 // Tile variants per Q (cells per workgroup) that the batched launch and its
