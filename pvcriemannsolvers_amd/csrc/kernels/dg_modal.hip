@@ -115,34 +115,62 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
   if (c0 >= a.n_cells) return;
   const int nc = (int)min<int64_t>(G, a.n_cells - c0);
 
+  // ---- this thread's point task and its global inputs: issued before the
+  // staging loads so that their latency overlaps them
+  const int g = tid / T, r = tid - (tid / T) * T;
+  const bool active = g < nc, is_face = r >= NQ;
+  const int lf = is_face ? (r - NQ) / NFP : 0, q = is_face ? (r - NQ) - ((r - NQ) / NFP) * NFP : r;
+  const DgTables *tb = a.tables;
+  int32_t nb = 0, info = 0;
+  double eta[3] = { 0, 0, 0 }, eta_nb[3] = { 0, 0, 0 }, wl = 0, fr[7] = { 0, 0, 0, 0, 0, 0, 0 };
+  if (active) {
+    if (!is_face) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) eta[d] = tb->vol_eta[r][d];
+      wl = tb->vol_w[r];
+    } else {
+      nb = a.nbr[(c0 + g) * 6 + lf];
+      info = a.finfo[(c0 + g) * 6 + lf];
+      const bool holder = (info >> 7) & 1;
+      const int orient = (info >> 3) & 15, nf = info & 7;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) eta[d] = tb->face_eta[lf][holder ? 0 : orient][q][d];
+      if (nb >= 0) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) eta_nb[d] = tb->face_eta[nf][holder ? orient : 0][q][d];
+      }
+      wl = tb->face_w[q];
+      const double *fc = a.faces + (int64_t)(info >> 8) * kDgFaceStride;
+#pragma unroll
+      for (int i = 0; i < 7; ++i) fr[i] = fc[i];
+    }
+  }
+
   // ---- phase 0: stage the cells' and their neighbours' data in LDS
   for (int w = tid; w < nc * 5 * N; w += 256) s_ct[w] = a.ct[c0 * 5 * N + w];
   for (int w = tid; w < nc * PK; w += 256) s_s[w] = a.basis[c0 * PK + w];
   for (int w = tid; w < nc * GS; w += 256) s_geo[w] = a.geo[c0 * GS + w];
   for (int w = tid; w < nc * NF * NB; w += 256) {
     const int gl = w / NB, e = w - (w / NB) * NB;
-    const int g = gl / NF, lf = gl - (gl / NF) * NF;
-    const int32_t nb = a.nbr[(c0 + g) * 6 + lf];
-    if (nb < 0 || e >= 5 * N + 9) continue;
-    s_nbr[w] = e < 5 * N ? a.ct[(int64_t)nb * 5 * N + e] : a.geo[(int64_t)nb * GS + (e - 5 * N)];
+    const int gg = gl / NF, lfn = gl - (gl / NF) * NF;
+    const int32_t nbn = a.nbr[(c0 + gg) * 6 + lfn];
+    if (nbn < 0 || e >= 5 * N + 9) continue;
+    s_nbr[w] = e < 5 * N ? a.ct[(int64_t)nbn * 5 * N + e] : a.geo[(int64_t)nbn * GS + (e - 5 * N)];
   }
   __syncthreads();
 
   // ---- phase 1: point tasks
-  const int g = tid / T, r = tid - (tid / T) * T;
-  if (g < nc) {
-    const int64_t cid = c0 + g;
+  if (active) {
     const double *geo = s_geo + g * GS;
     const double *own_ct = s_ct + g * 5 * N;
-    const DgTables *tb = a.tables;
-    if (r < NQ) {
+    if (!is_face) {
       double *slot = s_vol + (g * NQ + r) * kSlot;
       double y[3];
-      Offset(geo, tb->vol_eta[r], y);
+      Offset(geo, eta, y);
       const State u = EvalTaylor<P>(own_ct, y);
       double f[5][3];
       FluxMatrix(u, f);
-      const double w = __dmul_rn(tb->vol_w[r], geo[9]);
+      const double w = __dmul_rn(wl, geo[9]);
 #pragma unroll
       for (int k = 0; k < 5; ++k)
 #pragma unroll
@@ -150,30 +178,24 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) slot[15 + d] = y[d];
     } else {
-      const int lf = (r - NQ) / NFP, q = (r - NQ) - ((r - NQ) / NFP) * NFP;
       double *slot = s_face + (g * NF * NFP + (r - NQ)) * 9;
-      const int32_t nb = a.nbr[cid * 6 + lf];
-      const int32_t info = a.finfo[cid * 6 + lf];
       const bool holder = (info >> 7) & 1;
-      const int orient = (info >> 3) & 15;
-      const double *fc = a.faces + (int64_t)(info >> 8) * kDgFaceStride;
       FaceFrame frame;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) { frame.n[d] = fc[d]; frame.t[d] = fc[3 + d]; }
+      for (int d = 0; d < 3; ++d) { frame.n[d] = fr[d]; frame.t[d] = fr[3 + d]; }
       frame.b[0] = frame.n[1] * frame.t[2] - frame.n[2] * frame.t[1];
       frame.b[1] = frame.n[2] * frame.t[0] - frame.n[0] * frame.t[2];
       frame.b[2] = frame.n[0] * frame.t[1] - frame.n[1] * frame.t[0];
-      const double w = __dmul_rn(tb->face_w[q], fc[6]);
+      const double w = __dmul_rn(wl, fr[6]);
       double y[3];
-      Offset(geo, tb->face_eta[lf][holder ? 0 : orient][q], y);
+      Offset(geo, eta, y);
       const State own = EvalTaylor<P>(own_ct, y);
       State flux;
       double sgn = -1.0;
       if (nb >= 0) {
-        const int nf = info & 7;
         const double *rec = s_nbr + (g * NF + lf) * NB;
         double yn[3];
-        Offset(rec + 5 * N, tb->face_eta[nf][holder ? orient : 0][q], yn);
+        Offset(rec + 5 * N, eta_nb, yn);
         const State other = EvalTaylor<P>(rec, yn);
         flux = UpwindFlux<kRiemann>(frame, holder ? own : other, holder ? other : own);
         sgn = holder ? -1.0 : 1.0;
