@@ -170,12 +170,17 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
   const int ni = node / (Q * Q), nj = (node / Q) % Q, nk = node % Q;
   const int spos = ni * S::kSI + nj * S::kSJ + nk;  // skewed LDS position
   State cu, pu;
+  double u0v[5] = { 0, 0, 0, 0, 0 };
   const double *js = nullptr;
   double inv_det = 0;
   if (node_active) {
 #pragma unroll
     for (int k = 0; k < 5; ++k) cu.v[k] = a.us[base + tid * 5 + k];
     pu = FastPrimitive(cu);
+    if (a.stage >= 2) {  // U0 of the RK combination, fetched early
+#pragma unroll
+      for (int k = 0; k < 5; ++k) u0v[k] = a.u0[base + tid * 5 + k];
+    }
     js = a.geo + (int64_t)a.geo_id[c0 + ncell] * kGeoStride;
     inv_det = js[9];
 #pragma unroll
@@ -195,6 +200,14 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
     const int fp = r - (r / NFP) * NFP;
     t_a[t] = fp / Q;
     t_b[t] = fp - (fp / Q) * Q;
+  }
+  // neighbour and face-info words of the face tasks, fetched before the
+  // volume passes so that the face phase starts its gathers at once
+  int32_t t_nb[T], t_info[T];
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+    t_nb[t] = t_on[t] ? a.nbr[(c0 + t_cell[t]) * 6 + t_f[t]] : 0;
+    t_info[t] = t_on[t] ? a.finfo[(c0 + t_cell[t]) * 6 + t_f[t]] : 0;
   }
   double fn[T][5];
 
@@ -299,8 +312,8 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
         own.v[k] = Extrapolate<Q>(lw, vals);
       }
     }
-    const int32_t nb = a.nbr[cid * 6 + f];
-    const int32_t info = a.finfo[cid * 6 + f];
+    const int32_t nb = t_nb[t];
+    const int32_t info = t_info[t];
     const double *fr = a.frames + (int64_t)(info >> 7) * kFrameStride;
     FaceFrame frame;
 #pragma unroll
@@ -390,9 +403,9 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
         if (a.stage == 1) {
           out = next;
         } else if (a.stage == 2) {
-          out = __dadd_rn(__dmul_rn(a.u0[base + tid * 5 + k], 3.0), next) / 4.0;
+          out = __dadd_rn(__dmul_rn(u0v[k], 3.0), next) / 4.0;
         } else {
-          out = __dadd_rn(__dmul_rn(next, 2.0), a.u0[base + tid * 5 + k]) / 3.0;
+          out = __dadd_rn(__dmul_rn(next, 2.0), u0v[k]) / 3.0;
         }
       }
       a.out[base + tid * 5 + k] = out;
