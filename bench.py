@@ -322,8 +322,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-n", type=int, default=32)
     ap.add_argument("--cpu-steps", type=int, default=4)
-    ap.add_argument("--no-forms", action="store_true", help="skip the mixed and batched comparisons")
+    ap.add_argument("--no-forms", action="store_true", help="skip the mixed, batched and modal-DG forms")
+    ap.add_argument("--dg-only", action="store_true", help="run only the modal-DG forms (for profiling)")
     args = ap.parse_args()
+    if args.dg_only:
+        print(json.dumps({"dg_sod": bench_dg_sod(), "dg_tets_weno": bench_dg_tets()}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(sharding is described in DESIGN.md); use --gpus 1", file=sys.stderr)
