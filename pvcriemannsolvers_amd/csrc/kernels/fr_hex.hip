@@ -151,7 +151,7 @@ template <int Q, int Cells, int kRiemann, bool kDiag, bool kMixed>
 __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTables &lt, int64_t block,
                                                double *lds) {
   using S = Shape<Q, Cells>;
-  constexpr int NP = S::kPoints, NFP = S::kFacePoints, NT = S::kFaceTasks, C = S::kCells;
+  constexpr int NP = S::kPoints, NFP = S::kFacePoints, C = S::kCells;
   constexpr int PAD = S::kPad, FPAD = S::kFacePad, T = S::kTasks;
   double *s_u = lds;                          // states, SoA per cell
   double *s_x = lds + C * 5 * PAD;            // F~ column per pass, then flux gaps
@@ -188,16 +188,24 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
   }
 
   // ---- face task bookkeeping (flux points of the workgroup's cells)
+  // Tasks are numbered direction-major: w = ((d * C + cell) * 2 + side) * Q^2
+  // + a * Q + b covers the two faces normal to reference direction d.  A
+  // wavefront then holds flux points of one direction (of at most two where
+  // C * 2 Q^2 is not a multiple of 64), so the extrapolation of pass d runs
+  // only in the wavefronts that own direction d instead of in all of them.
+  constexpr int PER_DIR = C * 2 * NFP;
   int t_cell[T], t_f[T], t_a[T], t_b[T];
   bool t_on[T];
 #pragma unroll
   for (int t = 0; t < T; ++t) {
     const int w = tid + 256 * t;
-    t_on[t] = w < nc * NT;
-    const int cell = w / NT, r = w - (w / NT) * NT;
+    const int d = w / PER_DIR, rd = w - d * PER_DIR;
+    const int cell = rd / (2 * NFP), rc = rd - cell * (2 * NFP);
+    const int side = rc / NFP, fp = rc - side * NFP;
+    t_on[t] = d < 3 && cell < nc;
     t_cell[t] = cell;
-    t_f[t] = r / NFP;
-    const int fp = r - (r / NFP) * NFP;
+    // local face of (direction, side): x {4, 2}, y {1, 3}, z {0, 5}
+    t_f[t] = d == 0 ? (side ? 2 : 4) : (d == 1 ? (side ? 3 : 1) : (side ? 5 : 0));
     t_a[t] = fp / Q;
     t_b[t] = fp - (fp / Q) * Q;
   }
@@ -371,8 +379,7 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
 #pragma unroll
   for (int t = 0; t < T; ++t) {
     if (!t_on[t]) continue;
-    const int w = tid + 256 * t;
-    const int r = w - t_cell[t] * NT;
+    const int r = t_f[t] * NFP + t_a[t] * Q + t_b[t];  // face-major gap slot of the node phase
 #pragma unroll
     for (int k = 0; k < 5; ++k) s_x[t_cell[t] * S::kScratch + k * FPAD + r] = gap[t][k];
   }
