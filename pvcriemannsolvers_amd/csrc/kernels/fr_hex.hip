@@ -31,13 +31,15 @@ namespace mcfd {
 
 namespace {
 
-template <int Q, int Cells = 256 / (Q * Q * Q)>
+template <int Q, int Cells = 256 / (Q * Q * Q), int Threads = 256>
 struct Shape {
   static constexpr int kPoints = Q * Q * Q;        // solution points per cell
   static constexpr int kFacePoints = Q * Q;        // flux points per face
   static constexpr int kFaceTasks = 6 * Q * Q;     // flux points per cell
   static constexpr int kCells = Cells;             // cells per workgroup
-  static_assert(Cells * Q * Q * Q <= 256, "one thread per solution point");
+  static constexpr int kThreads = Threads;         // workgroup size
+  static_assert(Cells * Q * Q * Q <= Threads, "one thread per solution point");
+  static_assert(Threads % 64 == 0 && Threads <= 1024, "whole wavefronts");
   // Skewed LDS position of solution point (i, j, k): i * kSI + j * kSJ + k.
   // With kSJ = Q + 1 and kSI = Q (Q + 1) the Q x Q points of any line family
   // read by one wavefront fall in distinct LDS banks (Q = 4: 20 i + 5 j + k).
@@ -47,9 +49,10 @@ struct Shape {
   static constexpr int kPad = kSpan | 1;           // odd row stride per component
   static constexpr int kFacePad = kFaceTasks + 1;
   static constexpr int kScratch = 5 * (kPad > kFacePad ? kPad : kFacePad);
-  static constexpr int kTasks = (kCells * kFaceTasks + 255) / 256;  // per thread
-  static_assert(kCells >= 1, "at most 256 solution points per cell");
-  static constexpr int kLdsDoubles = kCells * 5 * kPad + kCells * kScratch + kCells * 5;
+  static constexpr int kTasks = (kCells * kFaceTasks + Threads - 1) / Threads;  // per thread
+  static_assert(kCells >= 1, "at most one workgroup of solution points per cell");
+  // states, F~ columns / flux gaps, mixed reference values, face area scales
+  static constexpr int kLdsDoubles = kCells * 5 * kPad + kCells * kScratch + kCells * 5 + kCells * 6;
 };
 
 // Line of solution points through in-face point (a, b) of local face f:
@@ -147,15 +150,16 @@ __device__ __forceinline__ void LocalFluxColumn(State const &c, State const &p, 
 // the divergence); each direction's FP32 partial is accumulated in FP64.
 // States, pressure recovery, traces, the Riemann solve, the flux gaps and
 // the RK combination stay FP64.
-template <int Q, int Cells, int kRiemann, bool kDiag, bool kMixed>
+template <int Q, int Cells, int Threads, int kRiemann, bool kDiag, bool kMixed>
 __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTables &lt, int64_t block,
                                                double *lds) {
-  using S = Shape<Q, Cells>;
+  using S = Shape<Q, Cells, Threads>;
   constexpr int NP = S::kPoints, NFP = S::kFacePoints, C = S::kCells;
   constexpr int PAD = S::kPad, FPAD = S::kFacePad, T = S::kTasks;
   double *s_u = lds;                          // states, SoA per cell
   double *s_x = lds + C * 5 * PAD;            // F~ column per pass, then flux gaps
   double *s_ref = s_x + C * S::kScratch;      // mixed: FP64 column value at node 0
+  double *s_scale = s_ref + C * 5;            // signed face area scales of each cell
   float *s_x32 = reinterpret_cast<float *>(s_x);
 
   const int tid = threadIdx.x;
@@ -183,6 +187,7 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
     }
     js = a.geo + (int64_t)a.geo_id[c0 + ncell] * kGeoStride;
     inv_det = js[9];
+    if (node < 6) s_scale[ncell * 6 + node] = js[10 + node];
 #pragma unroll
     for (int k = 0; k < 5; ++k) s_u[(ncell * 5 + k) * PAD + spos] = cu.v[k];
   }
@@ -198,7 +203,7 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
   bool t_on[T];
 #pragma unroll
   for (int t = 0; t < T; ++t) {
-    const int w = tid + 256 * t;
+    const int w = tid + Threads * t;
     const int d = w / PER_DIR, rd = w - d * PER_DIR;
     const int cell = rd / (2 * NFP), rc = rd - cell * (2 * NFP);
     const int side = rc / NFP, fp = rc - side * NFP;
@@ -305,7 +310,6 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
   for (int t = 0; t < T; ++t) {
     if (!t_on[t]) continue;
     const int f = t_f[t], ia = t_a[t], ib = t_b[t];
-    const int64_t cid = c0 + t_cell[t];
     const LineIndex li = SkewedLineIndex<Q>(f);
     const double *lw = lt.lend[li.side];
     State own;
@@ -329,7 +333,7 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
     frame.b[0] = frame.n[1] * frame.t[2] - frame.n[2] * frame.t[1];
     frame.b[1] = frame.n[2] * frame.t[0] - frame.n[0] * frame.t[2];
     frame.b[2] = frame.n[0] * frame.t[1] - frame.n[1] * frame.t[0];
-    const double scale = a.geo[(int64_t)a.geo_id[cid] * kGeoStride + 10 + f];
+    const double scale = s_scale[t_cell[t] * 6 + f];
     State fu;
     double sgn = 1.0;
     if (nb >= 0) {
@@ -390,6 +394,7 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
     const double *sg = s_x + ncell * S::kScratch;
     const double *su = s_u + ncell * 5 * PAD;
     const double dt = a.case_dt ? a.case_dt[a.cell_case[c0 + ncell]] : a.dt;
+    double outv[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
       const double *g = sg + k * FPAD;
@@ -415,14 +420,17 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
           out = __dadd_rn(__dmul_rn(next, 2.0), u0v[k]) / 3.0;
         }
       }
-      a.out[base + tid * 5 + k] = out;
+      outv[k] = out;
     }
+    // one basic block of stores: merged into 16-byte stores of the node record
+#pragma unroll
+    for (int k = 0; k < 5; ++k) a.out[base + tid * 5 + k] = outv[k];
   }
 }
 
-template <int Q, int Cells, int kRiemann, bool kDiag, bool kMixed>
-__global__ __launch_bounds__(256) void FrHexStageKernel(FrHexArgs a, LineTables lt) {
-  __shared__ double lds[Shape<Q, Cells>::kLdsDoubles];
+template <int Q, int Cells, int Threads, int kRiemann, bool kDiag, bool kMixed>
+__global__ __launch_bounds__(Threads) void FrHexStageKernel(FrHexArgs a, LineTables lt) {
+  __shared__ double lds[Shape<Q, Cells, Threads>::kLdsDoubles];
   // The line tables are indexed by lane-varying node indices: read from the
   // kernel arguments they would be vector loads on the critical path of every
   // pass, so they are staged in LDS once (the body's first barrier orders it).
@@ -432,11 +440,21 @@ __global__ __launch_bounds__(256) void FrHexStageKernel(FrHexArgs a, LineTables 
     reinterpret_cast<double *>(&s_lt)[threadIdx.x] = reinterpret_cast<const double *>(&lt)[threadIdx.x];
   int64_t block = LogicalBlock(a.xcd_remap);
   if (a.tile_order) block = a.tile_order[block];
-  FrHexStageBody<Q, Cells, kRiemann, kDiag, kMixed>(a, s_lt, block, lds);
+  FrHexStageBody<Q, Cells, Threads, kRiemann, kDiag, kMixed>(a, s_lt, block, lds);
 }
 
+// Single-mesh launch shape per Q: 256 threads, one per solution point.  At
+// Q = 4 the face tasks (96 per cell) then fill two slots in half of the
+// wavefronts; 96 threads per cell (one task each) measured slower on MI355X,
+// since the extra wavefronts idle through the volume passes and occupy
+// wave slots that other workgroups would fill.
+template <int Q> struct StageTile {
+  static constexpr int kCells = Shape<Q>::kCells;
+  static constexpr int kThreads = 256;
+};
+
 // Tile variants per Q (cells per workgroup) that the batched launch and its
-// autotuner choose from; variant 0 is the single-mesh default.
+// autotuner choose from.
 template <int Q, int V> struct TileOf;
 template <> struct TileOf<2, 0> { static constexpr int kCells = 32; };
 template <> struct TileOf<2, 1> { static constexpr int kCells = 16; };
@@ -455,7 +473,7 @@ constexpr int TileLds() { return Shape<Q, TileOf<Q, V>::kCells>::kLdsDoubles; }
 // instead of the batched kernel carrying the union of nine inlined bodies.
 template <int Q, int V, int kRiemann, bool kMixed>
 __device__ __noinline__ void BatchBody(const FrHexBatchGroup *g, int64_t block, double *lds) {
-  FrHexStageBody<Q, TileOf<Q, V>::kCells, kRiemann, true, kMixed>(g->args, g->lt, block, lds);
+  FrHexStageBody<Q, TileOf<Q, V>::kCells, 256, kRiemann, true, kMixed>(g->args, g->lt, block, lds);
 }
 
 template <int Q, int kRiemann, bool kMixed>
@@ -521,21 +539,21 @@ __global__ __launch_bounds__(256) void FrHexTimeStepKernel(const double *u, cons
 
 template <int Q, int R>
 hipError_t Launch(const FrHexArgs &args, const LineTables &lt, hipStream_t stream) {
-  constexpr int C = Shape<Q>::kCells;
+  constexpr int C = StageTile<Q>::kCells, W = StageTile<Q>::kThreads;
   int64_t blocks = (args.n_cells + C - 1) / C;
   if (blocks > 0x7fffffff) return hipErrorInvalidValue;
-  const dim3 grid((unsigned)blocks), block(256);
+  const dim3 grid((unsigned)blocks), block(W);
   if (args.mixed) {
     if (args.diag_metric) {
-      hipLaunchKernelGGL((FrHexStageKernel<Q, C, R, true, true>), grid, block, 0, stream, args, lt);
+      hipLaunchKernelGGL((FrHexStageKernel<Q, C, W, R, true, true>), grid, block, 0, stream, args, lt);
     } else {
-      hipLaunchKernelGGL((FrHexStageKernel<Q, C, R, false, true>), grid, block, 0, stream, args, lt);
+      hipLaunchKernelGGL((FrHexStageKernel<Q, C, W, R, false, true>), grid, block, 0, stream, args, lt);
     }
   } else {
     if (args.diag_metric) {
-      hipLaunchKernelGGL((FrHexStageKernel<Q, C, R, true, false>), grid, block, 0, stream, args, lt);
+      hipLaunchKernelGGL((FrHexStageKernel<Q, C, W, R, true, false>), grid, block, 0, stream, args, lt);
     } else {
-      hipLaunchKernelGGL((FrHexStageKernel<Q, C, R, false, false>), grid, block, 0, stream, args, lt);
+      hipLaunchKernelGGL((FrHexStageKernel<Q, C, W, R, false, false>), grid, block, 0, stream, args, lt);
     }
   }
   return hipGetLastError();
@@ -575,11 +593,11 @@ int FrHexTileLdsBytes(int q, int tile) {
 
 int FrHexStageTileCells(int q) {
   switch (q) {
-  case 2: return Shape<2>::kCells;
-  case 3: return Shape<3>::kCells;
-  case 4: return Shape<4>::kCells;
-  case 5: return Shape<5>::kCells;
-  case 6: return Shape<6>::kCells;
+  case 2: return StageTile<2>::kCells;
+  case 3: return StageTile<3>::kCells;
+  case 4: return StageTile<4>::kCells;
+  case 5: return StageTile<5>::kCells;
+  case 6: return StageTile<6>::kCells;
   default: return 0;
   }
 }
