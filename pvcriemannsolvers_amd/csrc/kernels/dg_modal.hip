@@ -30,7 +30,14 @@
 namespace mcfd {
 namespace {
 
-constexpr int kSlot = 18;  // doubles per task in LDS: 15 flux + 3 offset
+// LDS record of one volume task: the weighted flux matrix F[k][0..2] with a
+// zero fourth column, then Y = (y, 1).  A face task: the weighted flux (5),
+// then Y (4).  With the constant factor and the zero column in place, every
+// Taylor moment of degree <= 2 is two lane-indexed reads per point:
+//   grad(y_a y_b) . F[k] = F[k][a] Y[b] + F[k][b] Y[a],   T = Y[a] Y[b],
+// where a factor index 3 stands for the constant 1 (basis/taylor.hpp:204-208).
+constexpr int kSlot = 24;
+constexpr int kFaceSlot = 9;
 
 template <bool kHex, int P>
 struct DgShape {
@@ -71,27 +78,17 @@ __device__ __forceinline__ State EvalTaylor(const double *ct, const double *y) {
   return u;
 }
 
-// Taylor monomial j of degree <= 2 (basis/taylor.hpp:204-208) as the
-// product of two factors y_a y_b (index 0 stands for the constant 1), and its
-// gradient.  Branch-free: lanes of one wavefront hold different j.
+// Taylor monomial j of degree <= 2 (basis/taylor.hpp:204-208: 1, x, y, z,
+// xx, xy, xz, yy, yz, zz) as the product Y[a] Y[b] of two factors of
+// Y = (x, y, z, 1).
 struct MonomialCode {
-  int a, b;  // factor indices into (1, x, y, z)
+  int a, b;
 };
 __device__ __forceinline__ MonomialCode MonomialOf(int j) {
   MonomialCode m;
-  m.a = j == 0 ? 0 : (j <= 3 ? j : (j <= 6 ? 1 : (j <= 8 ? 2 : 3)));
-  m.b = j <= 3 ? 0 : (j == 4 ? 1 : (j == 5 || j == 7 ? 2 : 3));
+  m.a = j == 0 ? 3 : (j <= 3 ? j - 1 : (j <= 6 ? 0 : (j <= 8 ? 1 : 2)));
+  m.b = j <= 3 ? 3 : (j == 4 ? 0 : (j == 5 || j == 7 ? 1 : 2));
   return m;
-}
-__device__ __forceinline__ double Pick(int i, const double *y) {
-  return i == 1 ? y[0] : (i == 2 ? y[1] : (i == 3 ? y[2] : 1.0));
-}
-// T_j(y) and grad T_j(y): d(y_a y_b)/d y_d = [a == d] y_b + [b == d] y_a.
-__device__ __forceinline__ void Monomial(MonomialCode m, const double *y, double *t, double *g) {
-  const double fa = Pick(m.a, y), fb = Pick(m.b, y);
-  *t = fa * fb;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) g[d] = (m.a == d + 1 ? fb : 0.0) + (m.b == d + 1 ? fa : 0.0);
 }
 
 template <bool kHex, int P, int kRiemann>
@@ -104,9 +101,9 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
   __shared__ double s_s[G * PK];           // own Gram-Schmidt factors
   __shared__ double s_geo[G * GS];         // own geometry
   __shared__ double s_nbr[G * NF * NB];    // neighbours' coefficients and J; then the moments Rt
-  __shared__ double s_vol[G * NQ * kSlot]; // volume tasks: flux x weight (15), offset (3)
-  __shared__ double s_face[G * NF * NFP * 9];  // face tasks: -/+ w Fhat (5), offset (3), pad; then new coefficients
-  static_assert(G * 5 * N <= G * NF * NB && G * 5 * N <= G * NF * NFP * 9, "LDS aliasing");
+  __shared__ double s_vol[G * NQ * kSlot]; // volume tasks: flux x weight, zero column, Y
+  __shared__ double s_face[G * NF * NFP * kFaceSlot];  // face tasks: -/+ w Fhat (5), Y; then new coefficients
+  static_assert(G * 5 * N <= G * NF * NB && G * 5 * N <= G * NF * NFP * kFaceSlot, "LDS aliasing");
   double *s_rt = s_nbr;
   double *s_new = s_face;
 
@@ -116,9 +113,14 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
   const int nc = (int)min<int64_t>(G, a.n_cells - c0);
 
   // ---- this thread's point task and its global inputs: issued before the
-  // staging loads so that their latency overlaps them
-  const int g = tid / T, r = tid - (tid / T) * T;
-  const bool active = g < nc, is_face = r >= NQ;
+  // staging loads so that their latency overlaps them.  Tasks are numbered
+  // volume points of all G cells first, then face points, so that at most one
+  // wavefront holds both kinds and runs both branches.
+  const bool is_face = tid >= G * NQ;
+  const int tf = tid - G * NQ;
+  const int g = is_face ? tf / (NF * NFP) : tid / NQ;
+  const int r = is_face ? NQ + (tf - g * (NF * NFP)) : tid - g * NQ;
+  const bool active = g < nc && tid < G * T;
   const int lf = is_face ? (r - NQ) / NFP : 0, q = is_face ? (r - NQ) - ((r - NQ) / NFP) * NFP : r;
   const DgTables *tb = a.tables;
   int32_t nb = 0, info = 0;
@@ -150,12 +152,20 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
   for (int w = tid; w < nc * 5 * N; w += 256) s_ct[w] = a.ct[c0 * 5 * N + w];
   for (int w = tid; w < nc * PK; w += 256) s_s[w] = a.basis[c0 * PK + w];
   for (int w = tid; w < nc * GS; w += 256) s_geo[w] = a.geo[c0 * GS + w];
-  for (int w = tid; w < nc * NF * NB; w += 256) {
-    const int gl = w / NB, e = w - (w / NB) * NB;
-    const int gg = gl / NF, lfn = gl - (gl / NF) * NF;
-    const int32_t nbn = a.nbr[(c0 + gg) * 6 + lfn];
-    if (nbn < 0 || e >= 5 * N + 9) continue;
-    s_nbr[w] = e < 5 * N ? a.ct[(int64_t)nbn * 5 * N + e] : a.geo[(int64_t)nbn * GS + (e - 5 * N)];
+  constexpr int kNbIter = (G * NF * NB + 255) / 256;
+  int32_t nbn[kNbIter];
+#pragma unroll
+  for (int it = 0; it < kNbIter; ++it) {
+    const int w = tid + 256 * it;
+    const int gl = w / NB, gg = gl / NF, lfn = gl - (gl / NF) * NF;
+    nbn[it] = w < nc * NF * NB ? a.nbr[(c0 + gg) * 6 + lfn] : -1;
+  }
+#pragma unroll
+  for (int it = 0; it < kNbIter; ++it) {
+    const int w = tid + 256 * it;
+    const int e = w - (w / NB) * NB;
+    if (nbn[it] >= 0 && e < 5 * N + 9)
+      s_nbr[w] = e < 5 * N ? a.ct[(int64_t)nbn[it] * 5 * N + e] : a.geo[(int64_t)nbn[it] * GS + (e - 5 * N)];
   }
   __syncthreads();
 
@@ -172,13 +182,16 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
       FluxMatrix(u, f);
       const double w = __dmul_rn(wl, geo[9]);
 #pragma unroll
-      for (int k = 0; k < 5; ++k)
+      for (int k = 0; k < 5; ++k) {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) slot[3 * k + d] = f[k][d] * w;
+        for (int d = 0; d < 3; ++d) slot[4 * k + d] = f[k][d] * w;
+        slot[4 * k + 3] = 0.0;
+      }
 #pragma unroll
-      for (int d = 0; d < 3; ++d) slot[15 + d] = y[d];
+      for (int d = 0; d < 3; ++d) slot[20 + d] = y[d];
+      slot[23] = 1.0;
     } else {
-      double *slot = s_face + (g * NF * NFP + (r - NQ)) * 9;
+      double *slot = s_face + (g * NF * NFP + (r - NQ)) * kFaceSlot;
       const bool holder = (info >> 7) & 1;
       FaceFrame frame;
 #pragma unroll
@@ -220,6 +233,7 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
       for (int k = 0; k < 5; ++k) slot[k] = flux.v[k] * (sgn * w);
 #pragma unroll
       for (int d = 0; d < 3; ++d) slot[5 + d] = y[d];
+      slot[8] = 1.0;
     }
   }
   __syncthreads();
@@ -229,25 +243,19 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
   for (int w = tid; w < nc * 5 * N; w += 256) {
     const int gc = w / (5 * N), rem = w - gc * 5 * N, j = rem / 5, k = rem - (rem / 5) * 5;
     const MonomialCode m = MonomialOf(j);
-    const double *vb = s_vol + gc * NQ * kSlot;
-    const double *fb = s_face + gc * NF * NFP * 9;
+    const double *vf_a = s_vol + gc * NQ * kSlot + 4 * k + m.a, *vf_b = vf_a + (m.b - m.a);
+    const double *vy_a = s_vol + gc * NQ * kSlot + 20 + m.a, *vy_b = vy_a + (m.b - m.a);
+    const double *ff = s_face + gc * NF * NFP * kFaceSlot + k;
+    const double *fy_a = s_face + gc * NF * NFP * kFaceSlot + 5 + m.a, *fy_b = fy_a + (m.b - m.a);
     double acc = 0.0;
-#pragma unroll 2
+#pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      const double *sl = vb + q * kSlot;
-      double t, gr[3];
-      Monomial(m, sl + 15, &t, gr);
-      acc = __fma_rn(sl[3 * k], gr[0], acc);
-      acc = __fma_rn(sl[3 * k + 1], gr[1], acc);
-      acc = __fma_rn(sl[3 * k + 2], gr[2], acc);
+      acc = __fma_rn(vf_a[q * kSlot], vy_b[q * kSlot], acc);
+      acc = __fma_rn(vf_b[q * kSlot], vy_a[q * kSlot], acc);
     }
-#pragma unroll 2
-    for (int q = 0; q < NF * NFP; ++q) {
-      const double *sl = fb + q * 9;
-      double t, gr[3];
-      Monomial(m, sl + 5, &t, gr);
-      acc = __fma_rn(sl[k], t, acc);
-    }
+#pragma unroll
+    for (int q = 0; q < NF * NFP; ++q)
+      acc = __fma_rn(ff[q * kFaceSlot], fy_a[q * kFaceSlot] * fy_b[q * kFaceSlot], acc);
     s_rt[w] = acc;
   }
   __syncthreads();
