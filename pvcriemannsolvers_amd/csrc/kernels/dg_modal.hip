@@ -30,14 +30,17 @@
 namespace mcfd {
 namespace {
 
-// LDS record of one volume task: the weighted flux matrix F[k][0..2] with a
-// zero fourth column, then Y = (y, 1).  A face task: the weighted flux (5),
-// then Y (4).  With the constant factor and the zero column in place, every
-// Taylor moment of degree <= 2 is two lane-indexed reads per point:
-//   grad(y_a y_b) . F[k] = F[k][a] Y[b] + F[k][b] Y[a],   T = Y[a] Y[b],
-// where a factor index 3 stands for the constant 1 (basis/taylor.hpp:204-208).
-constexpr int kSlot = 24;
+// LDS record of one volume task: the weighted flux matrix as F[d][k] (15),
+// a zero, then Y = (y, 1) (4).  A face task: the weighted flux (5), then Y
+// (4).  With the constant factor and the zero in place, every Taylor moment
+// of degree <= 2 is two lane-indexed reads per point:
+//   grad(y_a y_b) . F[k] = F[a][k] Y[b] + F[b][k] Y[a],   T = Y[a] Y[b],
+// where a factor index 3 stands for the constant 1 (basis/taylor.hpp:204-208)
+// and F[3][k] reads the zero.
+constexpr int kSlot = 20;
+constexpr int kSlotZero = 15, kSlotY = 16;
 constexpr int kFaceSlot = 9;
+__device__ __forceinline__ int FluxIndex(int d, int k) { return d == 3 ? kSlotZero : 5 * d + k; }
 
 template <bool kHex, int P>
 struct DgShape {
@@ -98,7 +101,6 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
   constexpr int NB = 5 * N + 10;  // neighbour record: Taylor coefficients, J rows, pad
   constexpr int GS = kDgGeoStride;
   __shared__ double s_ct[G * 5 * N];       // own Taylor coefficients
-  __shared__ double s_s[G * PK];           // own Gram-Schmidt factors
   __shared__ double s_geo[G * GS];         // own geometry
   __shared__ double s_nbr[G * NF * NB];    // neighbours' coefficients and J; then the moments Rt
   __shared__ double s_vol[G * NQ * kSlot]; // volume tasks: flux x weight, zero column, Y
@@ -150,7 +152,6 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
 
   // ---- phase 0: stage the cells' and their neighbours' data in LDS
   for (int w = tid; w < nc * 5 * N; w += 256) s_ct[w] = a.ct[c0 * 5 * N + w];
-  for (int w = tid; w < nc * PK; w += 256) s_s[w] = a.basis[c0 * PK + w];
   for (int w = tid; w < nc * GS; w += 256) s_geo[w] = a.geo[c0 * GS + w];
   constexpr int kNbIter = (G * NF * NB + 255) / 256;
   int32_t nbn[kNbIter];
@@ -182,14 +183,13 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
       FluxMatrix(u, f);
       const double w = __dmul_rn(wl, geo[9]);
 #pragma unroll
-      for (int k = 0; k < 5; ++k) {
+      for (int k = 0; k < 5; ++k)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) slot[4 * k + d] = f[k][d] * w;
-        slot[4 * k + 3] = 0.0;
-      }
+        for (int d = 0; d < 3; ++d) slot[5 * d + k] = f[k][d] * w;
+      slot[kSlotZero] = 0.0;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) slot[20 + d] = y[d];
-      slot[23] = 1.0;
+      for (int d = 0; d < 3; ++d) slot[kSlotY + d] = y[d];
+      slot[kSlotY + 3] = 1.0;
     } else {
       double *slot = s_face + (g * NF * NFP + (r - NQ)) * kFaceSlot;
       const bool holder = (info >> 7) & 1;
@@ -243,8 +243,9 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
   for (int w = tid; w < nc * 5 * N; w += 256) {
     const int gc = w / (5 * N), rem = w - gc * 5 * N, j = rem / 5, k = rem - (rem / 5) * 5;
     const MonomialCode m = MonomialOf(j);
-    const double *vf_a = s_vol + gc * NQ * kSlot + 4 * k + m.a, *vf_b = vf_a + (m.b - m.a);
-    const double *vy_a = s_vol + gc * NQ * kSlot + 20 + m.a, *vy_b = vy_a + (m.b - m.a);
+    const double *vb = s_vol + gc * NQ * kSlot;
+    const double *vf_a = vb + FluxIndex(m.a, k), *vf_b = vb + FluxIndex(m.b, k);
+    const double *vy_a = vb + kSlotY + m.a, *vy_b = vb + kSlotY + m.b;
     const double *ff = s_face + gc * NF * NFP * kFaceSlot + k;
     const double *fy_a = s_face + gc * NF * NFP * kFaceSlot + 5 + m.a, *fy_b = fy_a + (m.b - m.a);
     double acc = 0.0;
@@ -263,7 +264,7 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
   // ---- phase 3: R = Rt S^T, RK stage, store
   for (int w = tid; w < nc * 5 * N; w += 256) {
     const int gc = w / (5 * N), rem = w - gc * 5 * N, i = rem / 5, k = rem - (rem / 5) * 5;
-    const double *srow = s_s + gc * PK + i * (i + 1) / 2;
+    const double *srow = a.basis + (c0 + gc) * PK + i * (i + 1) / 2;  // L1-resident, shared by the cell's items
     const double *rt = s_rt + gc * 5 * N;
     double res = 0.0;
     for (int j = 0; j <= i; ++j) res = __fma_rn(rt[5 * j + k], srow[j], res);
@@ -291,7 +292,7 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
   for (int w = tid; w < nc * 5 * N; w += 256) {
     const int gc = w / (5 * N), rem = w - gc * 5 * N, j = rem / 5, k = rem - (rem / 5) * 5;
     const double *sn = s_new + gc * 5 * N;
-    const double *sc = s_s + gc * PK;
+    const double *sc = a.basis + (c0 + gc) * PK;
     double acc = 0.0;
     for (int i = j; i < N; ++i) acc = __fma_rn(sn[5 * i + k], sc[i * (i + 1) / 2 + j], acc);
     a.ct_out[c0 * 5 * N + w] = acc;
