@@ -401,6 +401,7 @@ void DgGroup::Upload(hipStream_t stream) {
     nadj.Upload(h_nadj, stream);
     flags.Allocate(n_cells);
     Check(hipMemsetAsync(flags.ptr, 0, n_cells, stream), "memset");
+    troubled.Allocate(size_t(n_cells) + 1);
     proj.Allocate(size_t(n_cells) * 7 * 5 * n);
     beta.Allocate(size_t(n_cells) * 7 * 5);
     if (limiter >= MCFD_LIMITER_WENO_EIGEN) eig.Allocate(size_t(n_cells) * 6 * 5 * n);
@@ -412,7 +413,8 @@ void DgGroup::Limit(double *u, double *ct, hipStream_t stream) {
   DgLimiterArgs a;
   a.u = u; a.ct = ct; a.basis = basis.ptr; a.geo = geo.ptr;
   a.adj = adj.ptr; a.adj_face = adj_face.ptr; a.n_adj = nadj.ptr; a.faces = face_rec.ptr; a.tables = tables_d.ptr;
-  a.flags = flags.ptr; a.proj = proj.ptr; a.beta = beta.ptr; a.eig = eig.ptr;
+  a.flags = flags.ptr; a.troubled = troubled.ptr; a.n_troubled = troubled.ptr + n_cells;
+  a.proj = proj.ptr; a.beta = beta.ptr; a.eig = eig.ptr;
   a.n_cells = n_cells; a.mode = limiter; a.w0 = weno_w0; a.eps = weno_eps;
   Check(LaunchDgLimiter(cell_type, degree, a, stream), "DG limiter launch");
 }

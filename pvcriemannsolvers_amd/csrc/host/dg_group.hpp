@@ -41,6 +41,7 @@ struct DgGroup {
   std::vector<int32_t> h_adj, h_adj_face, h_nadj;  // Cell::adj_cells_/adj_faces_ in face order, [cell][6]
   DeviceArray<int32_t> adj, adj_face, nadj;
   DeviceArray<uint8_t> flags;
+  DeviceArray<int32_t> troubled;       // flagged cells of the last limiter pass, then their count
   DeviceArray<double> proj, beta, eig;
 
   void Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &scheme,

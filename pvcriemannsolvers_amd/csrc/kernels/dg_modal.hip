@@ -342,23 +342,35 @@ __global__ __launch_bounds__(256) void DgTimeStepKernel(const double *ct, const 
 // ---------------------------------------------------------------- WENO limiter
 constexpr int kLimiterSlots = 7;  // up to 6 neighbours, then the cell itself
 
-// u(c_self) from a cell's Taylor coefficients: Ct T(centre_self - centre_cell).
+// Components 0 (rho) and 4 (E) of Ct T(y): the same FMA chains as EvalTaylor.
 template <int P>
-__device__ __forceinline__ State ValueAtPoint(const double *ct, const double *geo_cell, const double *x) {
-  const double y[3] = { x[0] - geo_cell[11], x[1] - geo_cell[12], x[2] - geo_cell[13] };
-  return EvalTaylor<P>(ct, y);
+__device__ __forceinline__ void EvalTaylorRhoE(const double *ct, const double *y, double *rho, double *e) {
+  constexpr int N = P == 1 ? 4 : 10;
+  double t[10];
+  t[0] = 1.0; t[1] = y[0]; t[2] = y[1]; t[3] = y[2];
+  if constexpr (P >= 2) {
+    t[4] = __dmul_rn(y[0], y[0]); t[5] = __dmul_rn(y[0], y[1]); t[6] = __dmul_rn(y[0], y[2]);
+    t[7] = __dmul_rn(y[1], y[1]); t[8] = __dmul_rn(y[1], y[2]); t[9] = __dmul_rn(y[2], y[2]);
+  }
+  double a0 = ct[0], a4 = ct[4];
+#pragma unroll
+  for (int j = 1; j < N; ++j) {
+    a0 = __fma_rn(ct[5 * j], t[j], a0);
+    a4 = __fma_rn(ct[5 * j + 4], t[j], a4);
+  }
+  *rho = a0;
+  *e = a4;
 }
 
-// limiter/weno.hpp:230-258 (weno::Eigen::IsNotSmooth) on rho and E.
+__global__ void DgLimiterResetKernel(int32_t *n_troubled) { *n_troubled = 0; }
+
+// limiter/weno.hpp:230-258 (weno::Eigen::IsNotSmooth) on rho and E; every
+// flagged cell is also appended to the troubled-cell list (one atomic per
+// wavefront; the list order is irrelevant, each cell's reconstruction only
+// depends on the cell).
 template <int P>
-__global__ __launch_bounds__(256) void DgWenoFlagsKernel(DgLimiterArgs a) {
+__device__ __forceinline__ bool IsNotSmooth(const DgLimiterArgs &a, int64_t c) {
   constexpr int N = P == 1 ? 4 : 10, PACKED = N * (N + 1) / 2;
-  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (c >= a.n_cells) return;
-  if (a.mode == 2 || a.mode == 4) {
-    a.flags[c] = 1;
-    return;
-  }
   const double *gc = a.geo + c * kDgGeoStride;
   const double s00 = a.basis[c * PACKED];
   const double *cf = a.u + c * 5 * N;
@@ -369,18 +381,38 @@ __global__ __launch_bounds__(256) void DgWenoFlagsKernel(DgLimiterArgs a) {
   const int na = a.n_adj[c];
   for (int i = 0; i < na; ++i) {
     const int64_t o = a.adj[c * 6 + i];
-    const State theirs = ValueAtPoint<P>(a.ct + o * 5 * N, a.geo + o * kDgGeoStride, gc + 11);
+    const double *go = a.geo + o * kDgGeoStride;
+    const double y[3] = { gc[11] - go[11], gc[12] - go[12], gc[13] - go[13] };
+    double rho, e;
+    EvalTaylorRhoE<P>(a.ct + o * 5 * N, y, &rho, &e);
     const double so = a.basis[o * PACKED];
-    sum0 += fabs(mine0 - theirs.v[0]);
-    sum4 += fabs(mine4 - theirs.v[4]);
+    sum0 += fabs(mine0 - rho);
+    sum4 += fabs(mine4 - e);
     max_abs0 = fmax(max_abs0, fabs(a.u[o * 5 * N] * so));
     max_abs4 = fmax(max_abs4, fabs(a.u[o * 5 * N + 4] * so));
   }
   double divisor = pow(gc[14], (P + 1.0) / 2.0 / 3);
   divisor *= na;
   const double reference = P < 3 ? 1.0 : 3.0;
-  const bool rough = sum0 / max_abs0 / divisor > reference || sum4 / max_abs4 / divisor > reference;
-  a.flags[c] = rough ? 1 : 0;
+  return sum0 / max_abs0 / divisor > reference || sum4 / max_abs4 / divisor > reference;
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void DgWenoFlagsKernel(DgLimiterArgs a) {
+  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  bool rough = false;
+  if (c < a.n_cells) {
+    rough = (a.mode == 2 || a.mode == 4) ? true : IsNotSmooth<P>(a, c);
+    a.flags[c] = rough ? 1 : 0;
+  }
+  const unsigned long long ballot = __ballot(rough);
+  if (ballot == 0) return;
+  const int lane = (int)(threadIdx.x & 63);
+  const int leader = __ffsll((long long)ballot) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(a.n_troubled, __popcll(ballot));
+  base = __shfl(base, leader);
+  if (rough) a.troubled[base + __popcll(ballot & ((1ull << lane) - 1))] = (int32_t)c;
 }
 
 // weno.hpp:74-95 (GetSmoothness) of the scalar expansion c[stride * i]
@@ -435,66 +467,88 @@ __device__ double Smoothness(const double *c, int stride, const double *s, const
 // Slot i < n_adj projects neighbour i onto the cell's basis and shifts its
 // average onto the cell's; slot n_adj is the cell's own expansion.
 template <bool kHex, int P>
-__global__ __launch_bounds__(256) void DgWenoProjectKernel(DgLimiterArgs a) {
+__global__ __launch_bounds__(64) void DgWenoProjectKernel(DgLimiterArgs a) {
   constexpr int N = P == 1 ? 4 : 10, PACKED = N * (N + 1) / 2, NQ = DgShape<kHex, P>::kNQ;
-  const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int64_t c = w / kLimiterSlots;
-  const int slot = (int)(w - c * kLimiterSlots);
-  if (c >= a.n_cells || !a.flags[c]) return;
-  const int na = a.n_adj[c];
-  if (slot > na) return;
-  const double *gc = a.geo + c * kDgGeoStride;
-  const double *s = a.basis + c * PACKED;
+  constexpr int NA = kHex ? 6 : 4;          // neighbours at most
+  __shared__ double s_ct[NA][5 * N];        // neighbours' Taylor coefficients
+  __shared__ double s_shift[NA][3];         // own centre - neighbour centre
+  __shared__ double s_y[NQ][3];             // the cell's quadrature points, offsets from its centre
+  __shared__ double s_phi[NQ][N];           // its basis at them
+  __shared__ double s_w[NQ];                // and their global weights
+  __shared__ double s_f[NA][NQ][5];         // neighbour polynomials at them
+  __shared__ double s_pr[NA + 1][5 * N];    // projections, the cell's own last
+  const int lane = threadIdx.x;
   const DgTables *tb = a.tables;
-  double *pr = a.proj + (c * kLimiterSlots + slot) * 5 * N;
-  double acc[10][5];
-  if (slot == na) {
+  const int count = *a.n_troubled;
+  for (int li = blockIdx.x; li < count; li += gridDim.x) {
+    const int64_t c = a.troubled[li];
+    const int na = a.n_adj[c];
+    const double *gc = a.geo + c * kDgGeoStride;
+    const double *s = a.basis + c * PACKED;
+    for (int e = lane; e < na * 5 * N; e += 64) {
+      const int sl = e / (5 * N), r = e - sl * (5 * N);
+      s_ct[sl][r] = a.ct[(int64_t)a.adj[c * 6 + sl] * 5 * N + r];
+    }
+    if (lane < na) {
+      const double *go = a.geo + (int64_t)a.adj[c * 6 + lane] * kDgGeoStride;
 #pragma unroll
-    for (int i = 0; i < N; ++i)
-#pragma unroll
-      for (int k = 0; k < 5; ++k) acc[i][k] = a.u[c * 5 * N + 5 * i + k];
-  } else {
-    const int64_t o = a.adj[c * 6 + slot];
-    const double *go = a.geo + o * kDgGeoStride;
-    const double shift[3] = { gc[11] - go[11], gc[12] - go[12], gc[13] - go[13] };
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-#pragma unroll
-      for (int k = 0; k < 5; ++k) acc[i][k] = 0.0;
-    for (int q = 0; q < NQ; ++q) {  // projection.hpp:28-43 (Project)
+      for (int d = 0; d < 3; ++d) s_shift[lane][d] = gc[11 + d] - go[11 + d];
+    }
+    for (int q = lane; q < NQ; q += 64) {
       double y[3];
       Offset(gc, tb->vol_eta[q], y);
-      const double yo[3] = { y[0] + shift[0], y[1] + shift[1], y[2] + shift[2] };
-      const State f = EvalTaylor<P>(a.ct + o * 5 * N, yo);
       double t[10];
       t[0] = 1.0; t[1] = y[0]; t[2] = y[1]; t[3] = y[2];
       if constexpr (P >= 2) {
         t[4] = y[0] * y[0]; t[5] = y[0] * y[1]; t[6] = y[0] * y[2];
         t[7] = y[1] * y[1]; t[8] = y[1] * y[2]; t[9] = y[2] * y[2];
       }
-      const double wq = tb->vol_w[q] * gc[9];
 #pragma unroll
       for (int i = 0; i < N; ++i) {
         double phi = 0.0;
         for (int j = 0; j <= i; ++j) phi = __fma_rn(s[i * (i + 1) / 2 + j], t[j], phi);
+        s_phi[q][i] = phi;
+      }
 #pragma unroll
-        for (int k = 0; k < 5; ++k) acc[i][k] += (f.v[k] * phi) * wq;
+      for (int d = 0; d < 3; ++d) s_y[q][d] = y[d];
+      s_w[q] = tb->vol_w[q] * gc[9];
+    }
+    __syncthreads();
+    for (int e = lane; e < na * NQ; e += 64) {
+      const int sl = e / NQ, q = e - sl * NQ;
+      const double yo[3] = { s_y[q][0] + s_shift[sl][0], s_y[q][1] + s_shift[sl][1], s_y[q][2] + s_shift[sl][2] };
+      const State f = EvalTaylor<P>(s_ct[sl], yo);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) s_f[sl][q][k] = f.v[k];
+    }
+    __syncthreads();
+    // projection.hpp:28-43 (Project) and ProjectionWrapper::operator+=(my
+    // average - its average) on the constant coefficient
+    for (int e = lane; e < na * 5 * N; e += 64) {
+      const int sl = e / (5 * N), r = e - sl * (5 * N), i = r / 5, k = r - (r / 5) * 5;
+      double acc = 0.0;
+      for (int q = 0; q < NQ; ++q) acc += (s_f[sl][q][k] * s_phi[q][i]) * s_w[q];
+      if (i == 0) {
+        const double s00 = s[0];
+        acc += (a.u[c * 5 * N + k] * s00 - acc * s00) / s00;
+      }
+      s_pr[sl][r] = acc;
+      a.proj[(c * kLimiterSlots + sl) * 5 * N + r] = acc;
+    }
+    for (int r = lane; r < 5 * N; r += 64) {
+      const double v = a.u[c * 5 * N + r];
+      s_pr[na][r] = v;
+      a.proj[(c * kLimiterSlots + na) * 5 * N + r] = v;
+    }
+    if (a.mode < 3) {  // weno::Eigen measures smoothness in characteristic variables
+      __syncthreads();
+      for (int e = lane; e < (na + 1) * 5; e += 64) {
+        const int sl = e / 5, k = e - sl * 5;
+        a.beta[(c * kLimiterSlots + sl) * 5 + k] = Smoothness<kHex, P>(s_pr[sl] + k, 5, s, gc, tb);
       }
     }
-    const double s00 = s[0];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {  // ProjectionWrapper::operator+=(my average - its average)
-      const double offset = (a.u[c * 5 * N + k] * s00 - acc[0][k] * s00) / s00;
-      acc[0][k] += offset;
-    }
+    __syncthreads();  // the next cell reuses the LDS buffers
   }
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-#pragma unroll
-    for (int k = 0; k < 5; ++k) pr[5 * i + k] = acc[i][k];
-  if (a.mode >= 3) return;  // weno::Eigen measures smoothness in characteristic variables
-  for (int k = 0; k < 5; ++k)
-    a.beta[(c * kLimiterSlots + slot) * 5 + k] = Smoothness<kHex, P>(pr + k, 5, s, gc, tb);
 }
 
 // weno.hpp:157-194 (Lazy::Reconstruct): nonlinear weights and the weighted
@@ -504,9 +558,9 @@ template <int P>
 __global__ __launch_bounds__(256) void DgWenoCombineKernel(DgLimiterArgs a) {
   constexpr int N = P == 1 ? 4 : 10, PACKED = N * (N + 1) / 2;
   const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int64_t c = w / 5;
-  const int k = (int)(w - c * 5);
-  if (c >= a.n_cells || !a.flags[c]) return;
+  if (w >= (int64_t)*a.n_troubled * 5) return;
+  const int64_t c = a.troubled[w / 5];
+  const int k = (int)(w - (w / 5) * 5);
   const int na = a.n_adj[c];
   double wt[kLimiterSlots];
   for (int i = 0; i <= na; ++i) {
@@ -744,19 +798,21 @@ hipError_t LaunchDgLimiter(int cell_type, int degree, const DgLimiterArgs &a, hi
   const bool hex = cell_type == 0;
   const int n = degree == 1 ? 4 : 10;
   const int64_t b_flags = (a.n_cells + 255) / 256;
-  const int64_t b_proj = (a.n_cells * kLimiterSlots + 255) / 256;
+  // one 64-thread workgroup per troubled cell at a time, striding over the list
+  const int64_t b_proj = std::min<int64_t>(a.n_cells, 8192);
   const int64_t b_comb = (a.n_cells * 5 + 255) / 256;
   (void)n;
-  if (b_comb > 0x7fffffff || b_proj > 0x7fffffff) return hipErrorInvalidValue;
+  if (b_comb > 0x7fffffff) return hipErrorInvalidValue;
   const int64_t b_eig = (a.n_cells * 30 + 255) / 256;
   if (b_eig > 0x7fffffff) return hipErrorInvalidValue;
   const bool eigen = a.mode >= 3;
 #define MCFD_LIMITER_LAUNCH(P)                                                                          \
+  hipLaunchKernelGGL(DgLimiterResetKernel, dim3(1), dim3(1), 0, stream, a.n_troubled);                 \
   hipLaunchKernelGGL(DgWenoFlagsKernel<P>, dim3((unsigned)b_flags), dim3(256), 0, stream, a);           \
   if (hex) {                                                                                            \
-    hipLaunchKernelGGL((DgWenoProjectKernel<true, P>), dim3((unsigned)b_proj), dim3(256), 0, stream, a); \
+    hipLaunchKernelGGL((DgWenoProjectKernel<true, P>), dim3((unsigned)b_proj), dim3(64), 0, stream, a);  \
   } else {                                                                                              \
-    hipLaunchKernelGGL((DgWenoProjectKernel<false, P>), dim3((unsigned)b_proj), dim3(256), 0, stream, a); \
+    hipLaunchKernelGGL((DgWenoProjectKernel<false, P>), dim3((unsigned)b_proj), dim3(64), 0, stream, a); \
   }                                                                                                     \
   if (!eigen) {                                                                                         \
     hipLaunchKernelGGL(DgWenoCombineKernel<P>, dim3((unsigned)b_comb), dim3(256), 0, stream, a);        \

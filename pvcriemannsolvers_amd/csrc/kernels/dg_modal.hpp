@@ -73,6 +73,8 @@ struct DgLimiterArgs {
   const double *faces;    // face records, as DgModalArgs
   const DgTables *tables;
   uint8_t *flags;         // [cell]
+  int32_t *troubled;      // [cell] list of the flagged cells, in no particular order
+  int32_t *n_troubled;    // its length
   double *proj;           // scratch [cell][7][N][5]: neighbour projections, then the cell's own
   double *beta;           // scratch [cell][7][5]: their smoothness indicators (Lazy)
   double *eig;            // scratch [cell][6][5][N]: blended characteristic fields per face (Eigen)
