@@ -100,14 +100,17 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
   constexpr int N = S::kN, NQ = S::kNQ, NF = S::kNF, NFP = S::kNFP, T = S::kT, G = S::kG, PK = S::kPacked;
   constexpr int NB = 5 * N + 10;  // neighbour record: Taylor coefficients, J rows, pad
   constexpr int GS = kDgGeoStride;
-  __shared__ double s_ct[G * 5 * N];       // own Taylor coefficients
-  __shared__ double s_geo[G * GS];         // own geometry
+  __shared__ double s_own[G * 5 * N + G * GS];  // own Taylor coefficients and geometry; then S
   __shared__ double s_nbr[G * NF * NB];    // neighbours' coefficients and J; then the moments Rt
   __shared__ double s_vol[G * NQ * kSlot]; // volume tasks: flux x weight, zero column, Y
   __shared__ double s_face[G * NF * NFP * kFaceSlot];  // face tasks: -/+ w Fhat (5), Y; then new coefficients
   static_assert(G * 5 * N <= G * NF * NB && G * 5 * N <= G * NF * NFP * kFaceSlot, "LDS aliasing");
+  static_assert(G * PK <= G * 5 * N + G * GS, "LDS aliasing");
+  double *s_ct = s_own, *s_geo = s_own + G * 5 * N;
+  double *s_s = s_own;  // Gram-Schmidt factors, from phase 2 on
   double *s_rt = s_nbr;
   double *s_new = s_face;
+  constexpr int kSIter = (G * PK + 255) / 256, kItems = (G * 5 * N + 255) / 256;
 
   const int tid = threadIdx.x;
   const int64_t c0 = (int64_t)blockIdx.x * G;
@@ -238,8 +241,25 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
   }
   __syncthreads();
 
-  // ---- phase 2: Taylor moments Rt[k][j], items ordered cell-major so that
-  // the lanes of a wavefront read the same task records (LDS broadcast)
+  // ---- phase 2: the Gram-Schmidt factors S and the state and U0 of the
+  // phase-3 items are fetched first, so that their latency overlaps the
+  // moments; S goes to the freed LDS of the own data at the end
+  double s_pf[kSIter];
+#pragma unroll
+  for (int it = 0; it < kSIter; ++it) {
+    const int w = tid + 256 * it;
+    s_pf[it] = w < nc * PK ? a.basis[c0 * PK + w] : 0.0;
+  }
+  double us_pf[kItems], u0_pf[kItems];
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const int w = tid + 256 * it;
+    const bool on = w < nc * 5 * N;
+    us_pf[it] = on && a.stage >= 1 ? a.us[c0 * 5 * N + w] : 0.0;
+    u0_pf[it] = on && a.stage >= 2 ? a.u0[c0 * 5 * N + w] : 0.0;
+  }
+  // Taylor moments Rt[k][j], items ordered cell-major so that the lanes of a
+  // wavefront read the same task records (LDS broadcast)
   for (int w = tid; w < nc * 5 * N; w += 256) {
     const int gc = w / (5 * N), rem = w - gc * 5 * N, j = rem / 5, k = rem - (rem / 5) * 5;
     const MonomialCode m = MonomialOf(j);
@@ -259,12 +279,20 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
       acc = __fma_rn(ff[q * kFaceSlot], fy_a[q * kFaceSlot] * fy_b[q * kFaceSlot], acc);
     s_rt[w] = acc;
   }
+#pragma unroll
+  for (int it = 0; it < kSIter; ++it) {
+    const int w = tid + 256 * it;
+    if (w < nc * PK) s_s[w] = s_pf[it];
+  }
   __syncthreads();
 
   // ---- phase 3: R = Rt S^T, RK stage, store
-  for (int w = tid; w < nc * 5 * N; w += 256) {
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const int w = tid + 256 * it;
+    if (w >= nc * 5 * N) break;
     const int gc = w / (5 * N), rem = w - gc * 5 * N, i = rem / 5, k = rem - (rem / 5) * 5;
-    const double *srow = a.basis + (c0 + gc) * PK + i * (i + 1) / 2;  // L1-resident, shared by the cell's items
+    const double *srow = s_s + gc * PK + i * (i + 1) / 2;
     const double *rt = s_rt + gc * 5 * N;
     double res = 0.0;
     for (int j = 0; j <= i; ++j) res = __fma_rn(rt[5 * j + k], srow[j], res);
@@ -273,13 +301,13 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
     if (a.stage == 0) {
       out = res;
     } else {
-      const double next = __dadd_rn(a.us[idx], __dmul_rn(res, a.dt));
+      const double next = __dadd_rn(us_pf[it], __dmul_rn(res, a.dt));
       if (a.stage == 1) {
         out = next;
       } else if (a.stage == 2) {
-        out = __dadd_rn(__dmul_rn(a.u0[idx], 3.0), next) / 4.0;
+        out = __dadd_rn(__dmul_rn(u0_pf[it], 3.0), next) / 4.0;
       } else {
-        out = __dadd_rn(__dmul_rn(next, 2.0), a.u0[idx]) / 3.0;
+        out = __dadd_rn(__dmul_rn(next, 2.0), u0_pf[it]) / 3.0;
       }
     }
     a.out[idx] = out;
@@ -292,7 +320,7 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
   for (int w = tid; w < nc * 5 * N; w += 256) {
     const int gc = w / (5 * N), rem = w - gc * 5 * N, j = rem / 5, k = rem - (rem / 5) * 5;
     const double *sn = s_new + gc * 5 * N;
-    const double *sc = a.basis + (c0 + gc) * PK;
+    const double *sc = s_s + gc * PK;
     double acc = 0.0;
     for (int i = j; i < N; ++i) acc = __fma_rn(sn[5 * i + k], sc[i * (i + 1) / 2 + j], acc);
     a.ct_out[c0 * 5 * N + w] = acc;
@@ -368,8 +396,9 @@ __global__ void DgLimiterResetKernel(int32_t *n_troubled) { *n_troubled = 0; }
 // flagged cell is also appended to the troubled-cell list (one atomic per
 // wavefront; the list order is irrelevant, each cell's reconstruction only
 // depends on the cell).
-template <int P>
+template <bool kHex, int P>
 __device__ __forceinline__ bool IsNotSmooth(const DgLimiterArgs &a, int64_t c) {
+  constexpr int NA = kHex ? 6 : 4;  // neighbours at most
   constexpr int N = P == 1 ? 4 : 10, PACKED = N * (N + 1) / 2;
   const double *gc = a.geo + c * kDgGeoStride;
   const double s00 = a.basis[c * PACKED];
@@ -379,17 +408,26 @@ __device__ __forceinline__ bool IsNotSmooth(const DgLimiterArgs &a, int64_t c) {
   double sum0 = 0.0, sum4 = 0.0;
   const double mine0 = ct[0], mine4 = ct[4];  // u(centre) = Ct T(0)
   const int na = a.n_adj[c];
-  for (int i = 0; i < na; ++i) {
-    const int64_t o = a.adj[c * 6 + i];
+  // A fixed trip count lets every neighbour's loads issue at once; the
+  // unused slots read the cell itself and are not accumulated.
+  int64_t nbs[NA];
+#pragma unroll
+  for (int i = 0; i < NA; ++i) nbs[i] = i < na ? (int64_t)a.adj[c * 6 + i] : c;
+#pragma unroll
+  for (int i = 0; i < NA; ++i) {
+    const int64_t o = nbs[i];
     const double *go = a.geo + o * kDgGeoStride;
     const double y[3] = { gc[11] - go[11], gc[12] - go[12], gc[13] - go[13] };
     double rho, e;
     EvalTaylorRhoE<P>(a.ct + o * 5 * N, y, &rho, &e);
     const double so = a.basis[o * PACKED];
-    sum0 += fabs(mine0 - rho);
-    sum4 += fabs(mine4 - e);
-    max_abs0 = fmax(max_abs0, fabs(a.u[o * 5 * N] * so));
-    max_abs4 = fmax(max_abs4, fabs(a.u[o * 5 * N + 4] * so));
+    const double u0 = a.u[o * 5 * N], u4 = a.u[o * 5 * N + 4];
+    if (i < na) {
+      sum0 += fabs(mine0 - rho);
+      sum4 += fabs(mine4 - e);
+      max_abs0 = fmax(max_abs0, fabs(u0 * so));
+      max_abs4 = fmax(max_abs4, fabs(u4 * so));
+    }
   }
   double divisor = pow(gc[14], (P + 1.0) / 2.0 / 3);
   divisor *= na;
@@ -397,12 +435,12 @@ __device__ __forceinline__ bool IsNotSmooth(const DgLimiterArgs &a, int64_t c) {
   return sum0 / max_abs0 / divisor > reference || sum4 / max_abs4 / divisor > reference;
 }
 
-template <int P>
+template <bool kHex, int P>
 __global__ __launch_bounds__(256) void DgWenoFlagsKernel(DgLimiterArgs a) {
   const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
   bool rough = false;
   if (c < a.n_cells) {
-    rough = (a.mode == 2 || a.mode == 4) ? true : IsNotSmooth<P>(a, c);
+    rough = (a.mode == 2 || a.mode == 4) ? true : IsNotSmooth<kHex, P>(a, c);
     a.flags[c] = rough ? 1 : 0;
   }
   const unsigned long long ballot = __ballot(rough);
@@ -808,10 +846,11 @@ hipError_t LaunchDgLimiter(int cell_type, int degree, const DgLimiterArgs &a, hi
   const bool eigen = a.mode >= 3;
 #define MCFD_LIMITER_LAUNCH(P)                                                                          \
   hipLaunchKernelGGL(DgLimiterResetKernel, dim3(1), dim3(1), 0, stream, a.n_troubled);                 \
-  hipLaunchKernelGGL(DgWenoFlagsKernel<P>, dim3((unsigned)b_flags), dim3(256), 0, stream, a);           \
   if (hex) {                                                                                            \
+    hipLaunchKernelGGL((DgWenoFlagsKernel<true, P>), dim3((unsigned)b_flags), dim3(256), 0, stream, a);  \
     hipLaunchKernelGGL((DgWenoProjectKernel<true, P>), dim3((unsigned)b_proj), dim3(64), 0, stream, a);  \
   } else {                                                                                              \
+    hipLaunchKernelGGL((DgWenoFlagsKernel<false, P>), dim3((unsigned)b_flags), dim3(256), 0, stream, a); \
     hipLaunchKernelGGL((DgWenoProjectKernel<false, P>), dim3((unsigned)b_proj), dim3(64), 0, stream, a); \
   }                                                                                                     \
   if (!eigen) {                                                                                         \
