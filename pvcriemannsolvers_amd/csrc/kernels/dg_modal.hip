@@ -2,16 +2,18 @@
 //
 // Decomposition of one stage pass: a workgroup of 256 threads owns G cells;
 // each cell has T = NQ + NF * NFP point tasks (its volume quadrature points
-// and the quadrature points of its faces), one thread per task.
+// and the quadrature points of its faces), one thread per task, volume tasks
+// of all G cells first.
+//   phase 0  the cells' Taylor coefficients and geometry, and each
+//            neighbour's coefficients and Jacobian, are staged in LDS (one
+//            cooperative load per face, not per point);
 //   phase 1  volume task: state u = Ct T(y) at the point, flux matrix times
 //            the global weight -> LDS;  face task: own trace, neighbour trace
 //            (from the neighbour's Taylor coefficients and Jacobian),
-//            Riemann or boundary flux times -/+ the weight -> LDS.
-//   phase 0  the cells' Taylor coefficients, Gram-Schmidt factors and
-//            geometry, and each neighbour's coefficients and Jacobian, are
-//            staged in LDS (one cooperative load per face, not per point);
+//            Riemann or boundary flux times -/+ the weight -> LDS;
 //   phase 2  one thread per (cell, component, Taylor monomial) sums the
-//            moments Rt over the cell's tasks;
+//            moments Rt over the cell's tasks, while the Gram-Schmidt factors
+//            and the state / U0 of phase 3 arrive;
 //   phase 3  one thread per (cell, basis function, component): R = Rt S^T,
 //            the SSP-RK3 combination (temporal/rk.hpp:31-49 operation order)
 //            and the store;
