@@ -424,7 +424,7 @@ void DgGroup::SolutionChanged(hipStream_t stream) {
   Limit(u0.ptr, ct0.ptr, stream);  // spatial/with_limiter.hpp:56-59
 }
 
-void DgGroup::Stage(int stage, double dt, hipStream_t stream) {
+void DgGroup::Stage(int stage, double dt, hipStream_t stream, unsigned long long *dt_min_bits) {
   DgModalArgs a;
   a.u0 = u0.ptr;
   switch (stage) {
@@ -437,6 +437,7 @@ void DgGroup::Stage(int stage, double dt, hipStream_t stream) {
   a.nbr = nbr.ptr; a.finfo = finfo.ptr; a.faces = face_rec.ptr;
   a.bc_kind = bc_kind.ptr; a.bc_state = bc_state.ptr; a.tables = tables_d.ptr;
   a.n_cells = n_cells; a.dt = dt; a.stage = stage;
+  a.dt_min_bits = stage > 0 && !limiter ? dt_min_bits : nullptr;
   Check(LaunchDgModalStage(cell_type, degree, riemann, a, stream), "DG stage launch");
   if (stage > 0) Limit(a.out, a.ct_out, stream);  // spatial/with_limiter.hpp:56-59 after each SetSolutionColumn
 }

@@ -50,7 +50,9 @@ struct DgGroup {
 
   size_t ColumnSize() const { return size_t(n_cells) * n * 5; }
   // Stage 0 residual, 1..3 SSP-RK3 stages (Taylor coefficients, then the stage kernel).
-  void Stage(int stage, double dt, hipStream_t stream);
+  // With dt_min_bits set (no limiter only), the stage also folds the time-step
+  // reduction of its output state into it.
+  void Stage(int stage, double dt, hipStream_t stream, unsigned long long *dt_min_bits = nullptr);
   // Limiter pass over the state `u` with Taylor coefficients `ct`.
   void Limit(double *u, double *ct, hipStream_t stream);
   // The state u0 was set from outside: form its Taylor coefficients and limit it.

@@ -203,6 +203,7 @@ FrHexArgs FrHexGroup::Args(int stage, double dt) const {
   a.stage = stage;
   a.xcd_remap = 1;
   a.tile_order = tile_order.ptr;
+  a.dt_min_bits = nullptr;
   a.diag_metric = diag_metric ? 1 : 0;
   a.mixed = mixed ? 1 : 0;
   switch (stage) {

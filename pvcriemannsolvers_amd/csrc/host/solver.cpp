@@ -33,14 +33,29 @@ struct mcfd_solver {
   mcfd::FrHexGroup group;
   mcfd::DgGroup modal;
   mcfd::DeviceArray<unsigned long long> scratch;  // time-step reduction
+  // The time step (fem.hpp:412-435) of the state an RK step ends with can be
+  // folded into its stage-3 pass.  That is done while the caller reads a time
+  // step between steps (as the reference's driver does, sourceless.cpp:190-
+  // 193); a folded minimum left unread by the next step switches it off.
+  bool dt_fresh = false;    // scratch holds the minimum of the current solution
+  bool dt_pending = false;  // ... folded in by the last stage 3 and not read yet
+  bool fuse_dt_wanted = false;
 
   void Stage(int stage, double dt) {
+    if (stage == 1 && dt_pending) fuse_dt_wanted = false;
+    // the fused reduction needs the final state of the pass: not with a
+    // limiter, which rewrites troubled cells afterwards
+    const bool fuse_dt = stage == 3 && fuse_dt_wanted && !(dg && modal.limiter);
+    if (stage > 0) dt_fresh = dt_pending = false;
+    if (fuse_dt) Check(mcfd::LaunchResetTimeStepBits(scratch.ptr, stream), "time-step reset");
     if (dg) {
-      modal.Stage(stage, dt, stream);
-      return;
+      modal.Stage(stage, dt, stream, fuse_dt ? scratch.ptr : nullptr);
+    } else {
+      mcfd::FrHexArgs a = group.Args(stage, dt);
+      a.dt_min_bits = fuse_dt ? scratch.ptr : nullptr;
+      Check(mcfd::LaunchFrHexStage(group.q, group.riemann, a, group.lt, stream), "FR stage launch");
     }
-    Check(mcfd::LaunchFrHexStage(group.q, group.riemann, group.Args(stage, dt), group.lt, stream),
-          "FR stage launch");
+    if (fuse_dt) dt_fresh = dt_pending = true;
   }
   size_t ColumnSize() const { return dg ? modal.ColumnSize() : group.ColumnSize(); }
   double *Solution() { return dg ? modal.u0.ptr : group.u0.ptr; }
@@ -124,6 +139,7 @@ int mcfd_set_solution_column(mcfd_solver *h, const double *column) {
   MCFD_REQUIRE_DEVICE(h);
   if (!column) return Fail(MCFD_EINVAL, "null column");
   return Guard([&] {
+    h->dt_fresh = false;
     Check(hipMemcpyAsync(h->Solution(), column, h->ColumnSize() * sizeof(double),
                          hipMemcpyHostToDevice, h->stream), "set solution");
     if (h->dg) h->modal.SolutionChanged(h->stream);
@@ -176,15 +192,19 @@ int mcfd_get_time_step(mcfd_solver *h, double cfl, double dt_guess, double *dt) 
   MCFD_REQUIRE_DEVICE(h);
   if (!dt) return Fail(MCFD_EINVAL, "null output");
   return Guard([&] {
-    double init = 1e300, min_dt;
+    double min_dt;
     unsigned long long bits;
-    std::memcpy(&bits, &init, sizeof(bits));
-    Check(hipMemcpyAsync(h->scratch.ptr, &bits, sizeof(bits), hipMemcpyHostToDevice, h->stream), "dt init");
-    if (h->dg) {
-      h->modal.TimeStepBits(h->scratch.ptr, h->stream);
-    } else {
-      Check(mcfd::LaunchFrHexTimeStep(h->group.q, h->group.u0.ptr, h->group.geo_id.ptr, h->group.geo.ptr,
-                                      h->group.n_cells, h->scratch.ptr, h->stream), "time step launch");
+    h->fuse_dt_wanted = true;
+    h->dt_pending = false;
+    if (!h->dt_fresh) {  // no RK stage has folded it in since the solution last changed
+      Check(mcfd::LaunchResetTimeStepBits(h->scratch.ptr, h->stream), "time-step reset");
+      if (h->dg) {
+        h->modal.TimeStepBits(h->scratch.ptr, h->stream);
+      } else {
+        Check(mcfd::LaunchFrHexTimeStep(h->group.q, h->group.u0.ptr, h->group.geo_id.ptr, h->group.geo.ptr,
+                                        h->group.n_cells, h->scratch.ptr, h->stream), "time step launch");
+      }
+      h->dt_fresh = true;
     }
     Check(hipMemcpyAsync(&bits, h->scratch.ptr, sizeof(bits), hipMemcpyDeviceToHost, h->stream), "dt read");
     Check(hipStreamSynchronize(h->stream), "dt sync");

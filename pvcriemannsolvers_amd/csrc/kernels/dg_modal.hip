@@ -108,6 +108,7 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
   __shared__ double s_face[G * NF * NFP * kFaceSlot];  // face tasks: -/+ w Fhat (5), Y; then new coefficients
   static_assert(G * 5 * N <= G * NF * NB && G * 5 * N <= G * NF * NFP * kFaceSlot, "LDS aliasing");
   static_assert(G * PK <= G * 5 * N + G * GS, "LDS aliasing");
+  static_assert(G * N + G <= G * NQ * kSlot, "LDS aliasing");
   double *s_ct = s_own, *s_geo = s_own + G * 5 * N;
   double *s_s = s_own;  // Gram-Schmidt factors, from phase 2 on
   double *s_rt = s_nbr;
@@ -326,6 +327,31 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
     double acc = 0.0;
     for (int i = j; i < N; ++i) acc = __fma_rn(sn[5 * i + k], sc[i * (i + 1) / 2 + j], acc);
     a.ct_out[c0 * 5 * N + w] = acc;
+    s_rt[w] = acc;  // for the fused time step
+  }
+  if (!a.dt_min_bits) return;
+
+  // ---- phase 5: fem.hpp:412-435 on the new state, as DgTimeStepKernel: min
+  // over cells of length / max(1e-5, max over the first N points of |u| + c)
+  __syncthreads();
+  double *s_speed = s_vol, *s_cell = s_vol + G * N;
+  for (int w = tid; w < nc * N; w += 256) {
+    const int gc = w / N, q = w - gc * N;
+    double y[3];
+    Offset(a.geo + (c0 + gc) * GS, tb->vol_eta[q], y);
+    s_speed[w] = MaxSpeed(EvalTaylor<P>(s_rt + gc * 5 * N, y));
+  }
+  __syncthreads();
+  if (tid < nc) {
+    double vmax = 1e-5;
+    for (int q = 0; q < N; ++q) vmax = fmax(vmax, s_speed[tid * N + q]);
+    s_cell[tid] = a.geo[(c0 + tid) * GS + 10] / vmax;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double local = 1e300;
+    for (int c = 0; c < nc; ++c) local = fmin(local, s_cell[c]);
+    atomicMin(a.dt_min_bits, (unsigned long long)__double_as_longlong(local));
   }
 }
 

@@ -52,6 +52,9 @@ struct DgModalArgs {
   int64_t n_cells;
   double dt;
   int32_t stage;          // 0 residual, 1..3 SSP-RK3 stage
+  // If set (with ct_out), the time-step reduction of LaunchDgTimeStep over the
+  // state this pass writes is folded into *dt_min_bits (same arithmetic).
+  unsigned long long *dt_min_bits;
 };
 
 // Ct = C S for every cell of `us` into `ct`.

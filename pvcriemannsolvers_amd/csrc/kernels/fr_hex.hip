@@ -390,6 +390,7 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
   __syncthreads();
 
   // ---- node phase: lifted gaps, 1/det(J), RK stage, store
+  double speed = 0.0;  // |u| + c of the new state, for a fused time step
   if (node_active) {
     const double *sg = s_x + ncell * S::kScratch;
     const double *su = s_u + ncell * 5 * PAD;
@@ -425,6 +426,31 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
     // one basic block of stores: merged into 16-byte stores of the node record
 #pragma unroll
     for (int k = 0; k < 5; ++k) a.out[base + tid * 5 + k] = outv[k];
+    if (a.dt_min_bits) {
+      State nw;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) nw.v[k] = outv[k];
+      speed = MaxSpeed(nw);
+    }
+  }
+  if (a.dt_min_bits) {
+    // fem.hpp:412-435 on the state just written, as FrHexTimeStepKernel:
+    // min over cells of length / max(1e-5, max over nodes of |u| + c)
+    __syncthreads();  // every read of the LDS states and gaps is done
+    double *s_speed = s_u, *s_cell = s_x;
+    if (node_active) s_speed[tid] = speed;
+    __syncthreads();
+    if (tid < nc) {
+      double vmax = 1e-5;
+      for (int p = 0; p < NP; ++p) vmax = fmax(vmax, s_speed[tid * NP + p]);
+      s_cell[tid] = a.geo[(int64_t)a.geo_id[c0 + tid] * kGeoStride + 16] / vmax;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double local = 1e300;
+      for (int c = 0; c < nc; ++c) local = fmin(local, s_cell[c]);
+      atomicMin(a.dt_min_bits, (unsigned long long)__double_as_longlong(local));
+    }
   }
 }
 
@@ -509,6 +535,10 @@ __global__ __launch_bounds__(256) void FrHexBatchKernel(FrHexBatchArgs b) {
     }
     __syncthreads();  // the next item reuses the LDS buffers
   }
+}
+
+__global__ void ResetTimeStepBitsKernel(unsigned long long *bits) {
+  *bits = (unsigned long long)__double_as_longlong(1e300);
 }
 
 template <int Q>
@@ -641,6 +671,11 @@ int FrHexBatchResidentWorkgroups(int riemann, bool mixed, int lds_bytes) {
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return 0;
   return std::max(1, per_cu) * cus;
+}
+
+hipError_t LaunchResetTimeStepBits(unsigned long long *bits, hipStream_t stream) {
+  hipLaunchKernelGGL(ResetTimeStepBitsKernel, dim3(1), dim3(1), 0, stream, bits);
+  return hipGetLastError();
 }
 
 hipError_t LaunchFrHexTimeStep(int q, const double *u, const int32_t *geo_id, const double *geo,

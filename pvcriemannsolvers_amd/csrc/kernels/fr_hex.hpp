@@ -49,6 +49,9 @@ struct FrHexArgs {
   int32_t diag_metric;   // every J* is diagonal (axis-aligned cells)
   int32_t mixed;         // FP32 flux-column contractions (mixed-precision form)
   const int32_t *tile_order;  // [tile]: tile swept by logical block b (spatial locality), or null
+  // If set, the time-step reduction of LaunchFrHexTimeStep over the state this
+  // pass writes is folded into *dt_min_bits (same arithmetic, no extra pass).
+  unsigned long long *dt_min_bits;
 };
 
 // Launches one residual / RK-stage pass for Q points per direction and the
@@ -88,6 +91,8 @@ int FrHexBatchResidentWorkgroups(int riemann, bool mixed, int lds_bytes);
 
 // Per-cell minimum of length / max wave speed over the solution points
 // (spatial/fem.hpp:417-435), reduced to one double in *dt_min (device).
+// *bits = the ordered bits of 1e300, the start of a time-step reduction.
+hipError_t LaunchResetTimeStepBits(unsigned long long *bits, hipStream_t stream);
 hipError_t LaunchFrHexTimeStep(int q, const double *u, const int32_t *geo_id, const double *geo,
                                int64_t n_cells, unsigned long long *dt_min_bits, hipStream_t stream);
 

@@ -211,3 +211,23 @@ def test_limited_rk3_on_tets_matches_oracle(mode):
         np.testing.assert_array_equal(g.limiter_flags(), o.last_flags())
     assert rel(g.get_solution_column(), u) <= 1e-12
     g.close()
+
+
+@pytest.mark.parametrize("name", sorted(MESHES))
+def test_fused_time_step_matches_standalone(name):
+    """The time step folded into the last RK stage equals the standalone reduction bit for bit."""
+    mesh = MESHES[name]()
+    _, g = make_pair(mesh, 2, "hllc", MIXED_BCS)
+    ref = spatial.FiniteElement(mesh, degree=2, scheme="dg", riemann="hllc", bcs=MIXED_BCS)
+    g.approximate(smooth_state(g.sample_points()))
+    for step in range(6):
+        adaptive = step < 3  # then fixed-dt steps with no time step read in between
+        if adaptive or step == 5:
+            dt = g.get_time_step(0.2)
+            ref.set_solution_column(g.get_solution_column())
+            assert dt == ref.get_time_step(0.2)
+        g.rk3_update(0.0, dt)
+    ref.set_solution_column(g.get_solution_column())
+    assert g.get_time_step(0.2) == ref.get_time_step(0.2)
+    g.close()
+    ref.close()

@@ -220,3 +220,31 @@ def test_batched_cases_match_single_case_oracle():
     for i, o in enumerate(oracles):
         ref = o.rk3(states[i], dts[i], n_steps=3)
         assert rel(batch.get_solution_column(i), ref) < 1e-12, i
+
+
+@pytest.mark.parametrize("degree", [1, 3])
+def test_fused_time_step_matches_standalone(degree):
+    """The time step folded into stage 3 equals the standalone reduction bit for bit.
+
+    Reading a time step between steps (the reference driver's loop,
+    sourceless.cpp:190-193) makes stage 3 fold the reduction in; steps whose
+    minimum is left unread switch it off again.  Either way get_time_step
+    must return the fem.hpp:412-435 value of the current state.
+    """
+    mesh = meshgen.box_hex(5, 4, 3, hi=(1.0, 0.8, 0.6), shuffle_seed=7)
+    bcs = [("smart", conservative(1.0, (0.1, 0, 0), 1.0)), ("smart", conservative(0.125, (0, 0, 0), 0.1))] + \
+          [("inviscid_wall", None)] * 4
+    g = spatial.FiniteElement(mesh, degree=degree, riemann="hllc", bcs=bcs)
+    ref = spatial.FiniteElement(mesh, degree=degree, riemann="hllc", bcs=bcs)
+    g.set_solution_column(sod_like(g.sample_points()).reshape(-1))
+    for step in range(6):
+        adaptive = step < 3  # then three fixed-dt steps with no time step read in between
+        if adaptive or step == 5:
+            dt = g.get_time_step(0.1)
+            ref.set_solution_column(g.get_solution_column())
+            assert dt == ref.get_time_step(0.1)
+        g.rk3_update(0.0, dt)
+    ref.set_solution_column(g.get_solution_column())
+    assert g.get_time_step(0.1) == ref.get_time_step(0.1)
+    g.close()
+    ref.close()
