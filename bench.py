@@ -367,7 +367,7 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": read_traffic(f"fr_hex_stage_{n}cubed_p{p}_hbm_traffic.json"),
-                "kernel": "FrHexStageKernel<%d,2>" % (p + 1),
+                "kernel": "FrHexStageKernel (Q=%d, Roe, FP64)" % (p + 1),
                 "avg_kernel_ms": round(avg_kernel_s * 1e3, 4),
                 "stage_ms_p10_median_p90": [round(float(v), 4) for v in np.percentile(stage_ms, [10, 50, 90])],
                 "stages_timed": int(stage_ms.size),
