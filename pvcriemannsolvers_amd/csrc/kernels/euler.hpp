@@ -179,28 +179,42 @@ __device__ __forceinline__ double FastRcp(double x) {
 }
 
 // Conservative -> primitive with the reference's zeroing rule
-// (types.hpp:195-202,289-301), branch-free, via one reciprocal of rho.
-__device__ __forceinline__ State FastPrimitive(State const &c) {
-  bool bad = c.v[0] < 0 || c.v[4] < 0;
-  double rho = bad ? 0.0 : c.v[0];
-  double inv = rho != 0 ? FastRcp(rho) : 0.0;
+// (types.hpp:195-202,289-301) via one reciprocal of rho.  The plain
+// arithmetic is done first; the rule only selects among its results, so
+// a wavefront where no lane needs the rule (rho > 0, E >= 0, p >= 0 on every
+// lane) skips the selects and gets bitwise the same values as one that
+// applies them.  *inv_rho receives FastRcp of the returned density.
+__device__ __forceinline__ State FastPrimitive(State const &c, double *inv_rho) {
+  const double rho = c.v[0];
+  const double inv = FastRcp(rho);
   State p;
   p.v[0] = rho;
-  // rho == 0 keeps the momenta as they are, like the reference
-  p.v[1] = bad ? 0.0 : (rho != 0 ? c.v[1] * inv : c.v[1]);
-  p.v[2] = bad ? 0.0 : (rho != 0 ? c.v[2] * inv : c.v[2]);
-  p.v[3] = bad ? 0.0 : (rho != 0 ? c.v[3] * inv : c.v[3]);
-  double ke = 0.5 * (p.v[1] * p.v[1] + p.v[2] * p.v[2] + p.v[3] * p.v[3]);
-  double pr = rho != 0 ? (c.v[4] - rho * ke) * kGm1 : (bad ? 0.0 : c.v[4]);
-  bool bad2 = rho != 0 && pr < 0;
-  for (int i = 0; i < 4; ++i) p.v[i] = bad2 ? 0.0 : p.v[i];
-  p.v[4] = bad2 ? 0.0 : pr;
+  p.v[1] = c.v[1] * inv;
+  p.v[2] = c.v[2] * inv;
+  p.v[3] = c.v[3] * inv;
+  const double ke = 0.5 * (p.v[1] * p.v[1] + p.v[2] * p.v[2] + p.v[3] * p.v[3]);
+  p.v[4] = (c.v[4] - rho * ke) * kGm1;
+  *inv_rho = inv;
+  if (__all(rho > 0 && c.v[4] >= 0 && p.v[4] >= 0)) return p;
+  // rho < 0 or E < 0: all zero; rho == 0: momenta and E kept as they are;
+  // otherwise p < 0: all zero
+  const bool bad = c.v[0] < 0 || c.v[4] < 0;
+  const bool vacuum = !bad && c.v[0] == 0;
+  const bool zero = bad || (!vacuum && p.v[4] < 0);
+#pragma unroll
+  for (int k = 1; k < 5; ++k) p.v[k] = zero ? 0.0 : (vacuum ? c.v[k] : p.v[k]);
+  p.v[0] = zero ? 0.0 : rho;  // a vacuum keeps its signed zero
+  if (zero || vacuum) *inv_rho = FastRcp(p.v[0]);
   return p;
+}
+__device__ __forceinline__ State FastPrimitive(State const &c) {
+  double inv;
+  return FastPrimitive(c, &inv);
 }
 
 // Roe flux of normal-frame primitives (same waves as RoeFlux) with
-// reciprocals instead of divisions.
-__device__ __forceinline__ State FastRoeFlux(State const &l, State const &r) {
+// reciprocals instead of divisions; inv_l, inv_r = FastRcp of the densities.
+__device__ __forceinline__ State FastRoeFlux(State const &l, State const &r, double inv_l, double inv_r) {
   constexpr double kGoGm1 = kGamma / kGm1;
   double kel = 0.5 * (l.v[1] * l.v[1] + l.v[2] * l.v[2] + l.v[3] * l.v[3]);
   double ker = 0.5 * (r.v[1] * r.v[1] + r.v[2] * r.v[2] + r.v[3] * r.v[3]);
@@ -214,8 +228,8 @@ __device__ __forceinline__ State FastRoeFlux(State const &l, State const &r) {
   flux.v[4] = 0.5 * ((el + l.v[4]) * l.v[1] + (er + r.v[4]) * r.v[1]);
   double sl = sqrt(l.v[0]), sr = sqrt(r.v[0]);
   double inv = FastRcp(sl + sr);
-  double h0l = l.v[4] * FastRcp(l.v[0]) * kGoGm1 + kel;
-  double h0r = r.v[4] * FastRcp(r.v[0]) * kGoGm1 + ker;
+  double h0l = l.v[4] * inv_l * kGoGm1 + kel;
+  double h0r = r.v[4] * inv_r * kGoGm1 + ker;
   double rho = sl * sr;
   double u = (sl * l.v[1] + sr * r.v[1]) * inv;
   double v = (sl * l.v[2] + sr * r.v[2]) * inv;

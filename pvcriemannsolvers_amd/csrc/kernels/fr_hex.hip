@@ -106,10 +106,11 @@ __device__ __forceinline__ int64_t LogicalBlock(int32_t remap) {
 template <int kRiemann>
 __device__ __forceinline__ State FaceFlux(FaceFrame const &f, State const &cl, State const &cr) {
   if constexpr (kRiemann == kRoe) {
-    State l = FastPrimitive(cl), r = FastPrimitive(cr);
+    double inv_l, inv_r;
+    State l = FastPrimitive(cl, &inv_l), r = FastPrimitive(cr, &inv_r);
     GlobalToNormal(f, &l);
     GlobalToNormal(f, &r);
-    State flux = FastRoeFlux(l, r);
+    State flux = FastRoeFlux(l, r, inv_l, inv_r);
     NormalToGlobal(f, &flux);
     return flux;
   } else {
