@@ -278,7 +278,6 @@ void DgGroup::Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &scheme,
   h_geo.assign(n_cells * kDgGeoStride, 0.0);
   h_basis.assign(n_cells * packed, 0.0);
   qp.assign(n_cells * nq * 3, 0.0);
-  qw.assign(n_cells * nq, 0.0);
   std::vector<double> s(n * n);
   for (int64_t c = 0; c < n_cells; ++c) {
     Vec x[8];
@@ -309,7 +308,6 @@ void DgGroup::Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &scheme,
         qp[(c * nq + q) * 3 + d] = center[d] + y[q][d];
       }
       w[q] = tables.vol_w[q] * det;
-      qw[c * nq + q] = w[q];
       vol += w[q];
     }
     OrthoNormalize(degree, n, y, w, s.data());
@@ -447,30 +445,15 @@ void DgGroup::TimeStepBits(unsigned long long *dt_min_bits, hipStream_t stream) 
         "DG time-step launch");
 }
 
-void DgGroup::Approximate(const double *values, double *coeffs) const {
-  const int packed = n * (n + 1) / 2;
-  for (int64_t c = 0; c < n_cells; ++c) {
-    double *cf = coeffs + c * n * 5;
-    std::fill(cf, cf + n * 5, 0.0);
-    const double *s = h_basis.data() + c * packed;
-    const double *g = h_geo.data() + c * kDgGeoStride;
-    for (int q = 0; q < nq; ++q) {
-      double y[3];
-      const double *e = tables.vol_eta[q];
-      for (int d = 0; d < 3; ++d) y[d] = e[0] * g[d] + e[1] * g[3 + d] + e[2] * g[6 + d];
-      double t[10] = { 1, y[0], y[1], y[2], y[0] * y[0], y[0] * y[1], y[0] * y[2], y[1] * y[1], y[1] * y[2], y[2] * y[2] };
-      const double *f = values + (c * nq + q) * 5;
-      for (int i = 0; i < n; ++i) {
-        double phi = 0;
-        for (int jj = 0; jj <= i; ++jj) phi += s[i * (i + 1) / 2 + jj] * t[jj];
-        for (int k = 0; k < 5; ++k) {
-          double prod = f[k] * phi;
-          prod *= qw[c * nq + q];
-          cf[5 * i + k] += prod;
-        }
-      }
-    }
-  }
+void DgGroup::Approximate(const double *values, hipStream_t stream) {
+  DeviceArray<double> v;
+  v.Allocate(size_t(n_cells) * nq * 5);
+  Check(hipMemcpyAsync(v.ptr, values, size_t(n_cells) * nq * 5 * sizeof(double), hipMemcpyHostToDevice, stream),
+        "approximate upload");
+  Check(LaunchDgApproximate(degree, v.ptr, basis.ptr, geo.ptr, tables_d.ptr, nq, n_cells, u0.ptr, stream),
+        "DG approximate launch");
+  SolutionChanged(stream);
+  Check(hipStreamSynchronize(stream), "approximate sync");  // before the values buffer is freed
 }
 
 void DgGroup::Faces(int64_t *holder, int64_t *sharer, int32_t *holder_face, int32_t *sharer_face) const {

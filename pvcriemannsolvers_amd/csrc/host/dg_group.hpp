@@ -30,7 +30,7 @@ struct DgGroup {
   std::vector<DgTables> h_tables;     // upload source of `tables`
   std::vector<double> h_basis, h_geo, h_faces, h_bc_state;
   std::vector<int32_t> h_nbr, h_finfo, h_bc_kind;
-  std::vector<double> qp, qw;          // global quadrature points [cell][q][3] and weights [cell][q]
+  std::vector<double> qp;              // global quadrature points [cell][q][3]
   DeviceArray<double> u0, u1, u2, res, basis, geo, face_rec, bc_state;
   DeviceArray<double> ct0, ct1, ct2;  // Taylor coefficients of u0, u1, u2 (kept current)
   DeviceArray<int32_t> nbr, finfo, bc_kind;
@@ -59,8 +59,10 @@ struct DgGroup {
   void SolutionChanged(hipStream_t stream);
   // min over cells of length / max wave speed at the first N quadrature points.
   void TimeStepBits(unsigned long long *dt_min_bits, hipStream_t stream);
-  // Coefficients of f given at the quadrature points (polynomial/projection.hpp:28-43).
-  void Approximate(const double *values, double *coeffs) const;
+  // u0 = coefficients of f given at the quadrature points ([cell][nq][5] host
+  // array, polynomial/projection.hpp:28-43), projected on the device; then
+  // SolutionChanged.
+  void Approximate(const double *values, hipStream_t stream);
   void Faces(int64_t *holder, int64_t *sharer, int32_t *holder_face, int32_t *sharer_face) const;
 };
 

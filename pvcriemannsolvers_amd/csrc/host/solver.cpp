@@ -233,12 +233,10 @@ int mcfd_approximate(mcfd_solver *h, const double *values) {
     // (polynomial/hexahedron.hpp:646-652).
     return mcfd_set_solution_column(h, values);
   }
-  std::vector<double> coeffs;
-  int rc = Guard([&] {
-    coeffs.resize(h->ColumnSize());
-    h->modal.Approximate(values, coeffs.data());
+  return Guard([&] {
+    h->dt_fresh = false;
+    h->modal.Approximate(values, h->stream);  // projected on the device, then limited
   });
-  return rc != MCFD_OK ? rc : mcfd_set_solution_column(h, coeffs.data());
 }
 
 int mcfd_face_count(const mcfd_solver *h, int64_t *n_faces, int64_t *n_interior) {

@@ -355,6 +355,39 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
   }
 }
 
+// projection.hpp:28-43 (Project) of point values f at the cell quadrature
+// points onto the orthonormal basis, one thread per (cell, basis function,
+// component): c[i][k] = sum_q (f_q[k] phi_i(x_q)) w_q, with phi = S T(y) and the
+// operations in the order of the host-side projection.
+template <int P>
+__global__ __launch_bounds__(256) void DgApproximateKernel(const double *values, const double *basis,
+                                                           const double *geo, const DgTables *tb, int nq,
+                                                           int64_t n_cells, double *coeffs) {
+  constexpr int N = P == 1 ? 4 : 10, PACKED = N * (N + 1) / 2;
+  const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (w >= n_cells * 5 * N) return;
+  const int64_t c = w / (5 * N);
+  const int r = (int)(w - c * 5 * N), i = r / 5, k = r - (r / 5) * 5;
+  const double *g = geo + c * kDgGeoStride;
+  const double *s = basis + c * PACKED + i * (i + 1) / 2;
+  double acc = 0.0;
+  for (int q = 0; q < nq; ++q) {
+    const double *e = tb->vol_eta[q];
+    double y[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      y[d] = __dadd_rn(__dadd_rn(__dmul_rn(e[0], g[d]), __dmul_rn(e[1], g[3 + d])), __dmul_rn(e[2], g[6 + d]));
+    const double t[10] = { 1.0, y[0], y[1], y[2], __dmul_rn(y[0], y[0]), __dmul_rn(y[0], y[1]),
+                           __dmul_rn(y[0], y[2]), __dmul_rn(y[1], y[1]), __dmul_rn(y[1], y[2]),
+                           __dmul_rn(y[2], y[2]) };
+    double phi = 0.0;
+    for (int j = 0; j <= i; ++j) phi = __dadd_rn(phi, __dmul_rn(s[j], t[j]));
+    const double prod = __dmul_rn(__dmul_rn(values[(c * nq + q) * 5 + k], phi), __dmul_rn(tb->vol_w[q], g[9]));
+    acc = __dadd_rn(acc, prod);
+  }
+  coeffs[w] = acc;
+}
+
 template <int P>
 __global__ __launch_bounds__(256) void DgTaylorKernel(const double *us, const double *basis, int64_t n_cells,
                                                       double *ct) {
@@ -899,6 +932,24 @@ hipError_t LaunchDgLimiter(int cell_type, int degree, const DgLimiterArgs &a, hi
     return hipErrorInvalidValue;
   }
 #undef MCFD_LIMITER_LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t LaunchDgApproximate(int degree, const double *values, const double *basis, const double *geo,
+                               const DgTables *tables, int nq, int64_t n_cells, double *coeffs, hipStream_t stream) {
+  const int n = degree == 1 ? 4 : 10;
+  const int64_t blocks = (n_cells * 5 * n + 255) / 256;
+  if (blocks < 1) return hipSuccess;
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  if (degree == 1) {
+    hipLaunchKernelGGL(DgApproximateKernel<1>, dim3((unsigned)blocks), dim3(256), 0, stream, values, basis, geo,
+                       tables, nq, n_cells, coeffs);
+  } else if (degree == 2) {
+    hipLaunchKernelGGL(DgApproximateKernel<2>, dim3((unsigned)blocks), dim3(256), 0, stream, values, basis, geo,
+                       tables, nq, n_cells, coeffs);
+  } else {
+    return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

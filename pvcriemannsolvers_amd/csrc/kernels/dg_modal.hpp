@@ -87,6 +87,11 @@ struct DgLimiterArgs {
 };
 hipError_t LaunchDgLimiter(int cell_type, int degree, const DgLimiterArgs &a, hipStream_t stream);
 
+// Modal coefficients [cell][N][5] of point values [cell][nq][5] given at the
+// cell quadrature points (projection.hpp:28-43).
+hipError_t LaunchDgApproximate(int degree, const double *values, const double *basis, const double *geo,
+                               const DgTables *tables, int nq, int64_t n_cells, double *coeffs, hipStream_t stream);
+
 // min over cells of length / max_{q < N} (|u| + c)(x_q), as ordered bits of a
 // positive double folded into *dt_min_bits with atomicMin.
 hipError_t LaunchDgTimeStep(int cell_type, int degree, const double *ct, const double *geo,
