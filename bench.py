@@ -329,8 +329,7 @@ def main():
         print(json.dumps({"dg_sod": bench_dg_sod(), "dg_tets_weno": bench_dg_tets()}), flush=True)
         return
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(sharding is described in DESIGN.md); use --gpus 1", file=sys.stderr)
+        print("bench.py runs on one GPU only: the library has no multi-GPU path; use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     from pvcriemannsolvers_amd import meshgen, spatial
