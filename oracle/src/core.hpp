@@ -20,7 +20,8 @@
 //     (test/riemann/euler/hllc.cpp);
 //   * Roe: consistency, exact resolution of an isolated Rankine-Hugoniot
 //     shock and of a stationary contact, mirror symmetry.  No reference
-//     golden vectors exist (python/riemann.py is 1-D only and was not run).
+//     golden vectors exist (python/riemann.py is 1-D only and was not run),
+//     so Roe flux values are parity unpinned beyond these invariants.
 #ifndef ORACLE_CORE_HPP_
 #define ORACLE_CORE_HPP_
 
