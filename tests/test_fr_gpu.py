@@ -275,3 +275,21 @@ def test_zeroing_rule_on_nonphysical_nodes(riemann):
     g.set_solution_column(u.reshape(-1))
     assert rel(g.get_residual_column(), ref.reshape(-1)) < TOL[riemann]
     g.close()
+
+
+@pytest.mark.parametrize("degree", [3, 5])
+@pytest.mark.parametrize("riemann", ["roe", "hllc"])
+def test_partial_tile_and_largest_degree(degree, riemann):
+    """45 cells leave the last workgroup of 4 cells (Q = 4) partly empty; Q = 6
+    runs one cell of 216 solution points per workgroup."""
+    mesh = meshgen.box_hex(5, 3, 3, periodic=(True, True, True), shuffle_seed=9)
+    o, g = make_pair(mesh, degree, riemann)
+    u = density_wave(o.node_coords(), seed=4)
+    g.set_solution_column(u.reshape(-1))
+    assert rel(g.get_residual_column(), o.residual(u).reshape(-1)) < TOL[riemann]
+    dt = o.time_step(u.reshape(-1), 0.1)
+    ref = o.rk3(u.reshape(-1), dt, n_steps=2)
+    for s in range(2):
+        g.rk3_update(s * dt, dt)
+    assert rel(g.get_solution_column(), ref) < 1e-12
+    g.close()
