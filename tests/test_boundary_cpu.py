@@ -229,3 +229,31 @@ def test_dg_connectivity_and_points_match_oracle(name, degree):
     c = o.cells()
     np.testing.assert_allclose(fe.sample_points(), c["xyz"], rtol=0, atol=1e-14)
     fe.close()
+
+
+def test_unsupported_configurations_are_rejected():
+    """Configurations outside the path fail at creation with a status code, not later on the device."""
+    walls = [("inviscid_wall", None)] * 6
+    trilinear = meshgen.box_hex(2, 2, 2)
+    trilinear["coords"] = trilinear["coords"].copy()
+    trilinear["coords"][13] += (0.05, -0.03, 0.02)  # the interior vertex of the 2 x 2 x 2 box
+    cases = [
+        # the WENO limiter needs the modal DG scheme
+        (dict(mesh=meshgen.box_hex(2, 2, 2), degree=2, scheme="fr", bcs=walls, limiter="weno"),
+         C.MCFD_ENOTSUP, "DG_GENERAL"),
+        # and no neighbour polynomial is defined across periodic faces
+        (dict(mesh=meshgen.box_hex(3, 3, 3, periodic=(True,) * 3), degree=2, scheme="dg", limiter="weno"),
+         C.MCFD_EINVAL, "periodic"),
+        # modal DG carries p <= 2
+        (dict(mesh=meshgen.box_tet(2, 2, 2), degree=3, scheme="dg", bcs=walls), C.MCFD_EINVAL, "degree"),
+        # FR carries Q <= 6 points per direction
+        (dict(mesh=meshgen.box_hex(2, 2, 2), degree=6, scheme="fr", bcs=walls), C.MCFD_EINVAL, "degree"),
+        # trilinear (non-parallelepiped) hexahedra
+        (dict(mesh=trilinear, degree=2, bcs=walls), C.MCFD_EINVAL, "parallelepiped"),
+    ]
+    for kw, code, words in cases:
+        mesh = kw.pop("mesh")
+        with pytest.raises(C.MinicfdError) as e:
+            spatial.FiniteElement(mesh, host_only=True, **kw)
+        assert e.value.code == code, kw
+        assert words in C.load().mcfd_last_error().decode(), kw
