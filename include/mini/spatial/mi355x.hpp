@@ -1,0 +1,157 @@
+// mini::spatial::mi355x::FiniteElement: a mini::temporal::System<double>
+// whose residual and stages run on one MI355X through libminicfd.
+//
+// It takes the place of the reference's FiniteElement family behind the
+// System contract (spatial/fem.hpp:48,224-258): fr::General
+// (spatial/fr/general.hpp:218-231), dg::General (spatial/dg/general.hpp:38-40)
+// and WithLimiter (spatial/with_limiter.hpp:40-66).  Columns keep the
+// reference layout column[(cell * N + i) * 5 + k].  Boundary conditions are
+// given per boundary tag at creation (constant states, the SetSmartBoundary /
+// SetInviscidWall / SetSupersonic* registrations of fem.hpp:196-222).  Errors
+// of the C ABI are thrown as std::runtime_error carrying its message.
+#ifndef MINI_SPATIAL_MI355X_HPP_
+#define MINI_SPATIAL_MI355X_HPP_
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "mini/temporal/ode.hpp"
+#include "minicfd.h"
+
+namespace mini {
+namespace spatial {
+namespace mi355x {
+
+inline void Check(int rc) {
+  if (rc != MCFD_OK) throw std::runtime_error(std::string("libminicfd: ") + mcfd_last_error());
+}
+
+// fem.hpp:196-222 as constant states per boundary tag
+inline mcfd_bc_desc SmartBoundary(double const (&state)[5]) {
+  mcfd_bc_desc bc{ MCFD_BC_SMART, 0, { state[0], state[1], state[2], state[3], state[4] } };
+  return bc;
+}
+inline mcfd_bc_desc SupersonicInlet(double const (&state)[5]) {
+  mcfd_bc_desc bc{ MCFD_BC_SUPERSONIC_INLET, 0, { state[0], state[1], state[2], state[3], state[4] } };
+  return bc;
+}
+inline mcfd_bc_desc InviscidWall() { return mcfd_bc_desc{ MCFD_BC_INVISCID_WALL, 0, { 0, 0, 0, 0, 0 } }; }
+inline mcfd_bc_desc SupersonicOutlet() { return mcfd_bc_desc{ MCFD_BC_SUPERSONIC_OUTLET, 0, { 0, 0, 0, 0, 0 } }; }
+
+class FiniteElement : public temporal::System<double> {
+ public:
+  using Column = temporal::System<double>::Column;
+
+  FiniteElement(mcfd_mesh_desc const &mesh, mcfd_scheme_desc const &scheme, std::vector<mcfd_bc_desc> const &bcs) {
+    Check(mcfd_create(&mesh, &scheme, bcs.data(), (int32_t)bcs.size(), &h_));
+    Check(mcfd_sizes(h_, &n_cells_, &n_per_cell_, &size_));
+  }
+  FiniteElement(FiniteElement const &) = delete;
+  FiniteElement &operator=(FiniteElement const &) = delete;
+  FiniteElement(FiniteElement &&that) noexcept
+      : h_(std::exchange(that.h_, nullptr)), n_cells_(that.n_cells_), n_per_cell_(that.n_per_cell_),
+        size_(that.size_) {}
+  ~FiniteElement() override {
+    if (h_) mcfd_destroy(h_);
+  }
+
+  // fr::General<Part, rotated::Euler<Riemann>> with the Vincent correction
+  // c_next (1 = DG, p/(2p+1) = Huynh g2) on Gauss-Legendre or Lobatto nodes
+  template <class Riemann>
+  static FiniteElement FluxReconstruction(mcfd_mesh_desc const &mesh, int degree,
+                                          std::vector<mcfd_bc_desc> const &bcs, int nodes = MCFD_NODES_LEGENDRE,
+                                          double c_next = 1.0, int precision = MCFD_PRECISION_FP64) {
+    mcfd_scheme_desc s = Scheme(MCFD_SCHEME_FR_GENERAL, degree, Riemann::kId);
+    s.nodes = nodes;
+    s.c_next = c_next;
+    s.precision = precision;
+    return FiniteElement(mesh, s, bcs);
+  }
+  // dg::General<Part<..., Projection<double, 3, p, 5>>, rotated::Euler<Riemann>>
+  template <class Riemann>
+  static FiniteElement DiscontinuousGalerkin(mcfd_mesh_desc const &mesh, int degree,
+                                             std::vector<mcfd_bc_desc> const &bcs) {
+    return FiniteElement(mesh, Scheme(MCFD_SCHEME_DG_GENERAL, degree, Riemann::kId), bcs);
+  }
+  // WithLimiter<dg::General<...>, Limiter>: the limiter is applied after every
+  // SetSolutionColumn, Approximate and fused stage (with_limiter.hpp:56-59)
+  template <class Riemann, class Limiter>
+  static FiniteElement WithLimiter(mcfd_mesh_desc const &mesh, int degree, std::vector<mcfd_bc_desc> const &bcs,
+                                   bool all_cells = false, double w0 = 0.001, double eps = 1e-6) {
+    mcfd_scheme_desc s = Scheme(MCFD_SCHEME_DG_GENERAL, degree, Riemann::kId);
+    s.limiter = all_cells ? Limiter::kAll : Limiter::kFlagged;
+    s.weno_w0 = w0;
+    s.weno_eps = eps;
+    return FiniteElement(mesh, s, bcs);
+  }
+
+  // ---- temporal::System (ode.hpp:17-48, fem.hpp:224-258)
+  void SetTime(double t_curr) override { Check(mcfd_set_time(h_, t_curr)); }
+  void SetSolutionColumn(Column const &column) override {
+    if ((int64_t)column.size() != size_) throw std::runtime_error("column size mismatch");
+    Check(mcfd_set_solution_column(h_, column.data()));
+  }
+  Column GetSolutionColumn() const override {
+    Column column(size_);
+    Check(mcfd_get_solution_column(h_, column.data()));
+    return column;
+  }
+  Column GetResidualColumn() const override {
+    Column column(size_);
+    Check(mcfd_get_residual_column(h_, column.data()));
+    return column;
+  }
+
+  // ---- the rest of the FiniteElement surface used by the drivers
+  // GetTimeStep(dt_guess, rk_order) of fem.hpp:417-435 with an explicit CFL
+  double GetTimeStep(double cfl, double dt_guess = 1e300) const {
+    double dt = 0;
+    Check(mcfd_get_time_step(h_, cfl, dt_guess, &dt));
+    return dt;
+  }
+  // Approximate(f) of fem.hpp:147-154: values of f at SamplePoints()
+  std::vector<double> SamplePoints() const {
+    int32_t n = 0;
+    Check(mcfd_sample_points(h_, &n, nullptr));
+    std::vector<double> xyz((size_t)n_cells_ * n * 3);
+    Check(mcfd_sample_points(h_, &n, xyz.data()));
+    return xyz;
+  }
+  void Approximate(std::vector<double> const &values) { Check(mcfd_approximate(h_, values.data())); }
+  // RungeKutta<3>::Update (rk.hpp:31-49) fused on the device; the state stays in HBM
+  void Rk3Update(double t_curr, double dt) { Check(mcfd_rk3_update(h_, t_curr, dt)); }
+
+  int64_t CountCells() const { return n_cells_; }
+  int32_t UnknownsPerCell() const { return n_per_cell_; }
+  int64_t ColumnSize() const { return size_; }
+  mcfd_solver *handle() const { return h_; }
+
+ private:
+  static mcfd_scheme_desc Scheme(int32_t scheme, int degree, int32_t riemann) {
+    mcfd_scheme_desc s{};
+    s.scheme = scheme;
+    s.degree = degree;
+    s.nodes = MCFD_NODES_LEGENDRE;
+    s.riemann = riemann;
+    s.precision = MCFD_PRECISION_FP64;
+    s.c_next = 1.0;
+    s.limiter = MCFD_LIMITER_NONE;
+    s.weno_w0 = 0.001;
+    s.weno_eps = 1e-6;
+    return s;
+  }
+
+  mcfd_solver *h_ = nullptr;
+  int64_t n_cells_ = 0;
+  int32_t n_per_cell_ = 0;
+  int64_t size_ = 0;
+};
+
+}  // namespace mi355x
+}  // namespace spatial
+}  // namespace mini
+
+#endif  // MINI_SPATIAL_MI355X_HPP_

@@ -59,7 +59,7 @@ struct DgShape {
 // coordinates xi = xi_c + eta (J rows are d/dxi, d/deta, d/dzeta).
 __device__ __forceinline__ void Offset(const double *j, const double *eta, double *y) {
 #pragma unroll
-  for (int d = 0; d < 3; ++d) y[d] = __fma_rn(eta[2], j[6 + d], __fma_rn(eta[1], j[3 + d], __dmul_rn(eta[0], j[d])));
+  for (int d = 0; d < 3; ++d) y[d] = __fma_rn(eta[2], j[6 + d], __fma_rn(eta[1], j[3 + d], MulRn(eta[0], j[d])));
 }
 
 // u = Ct T(y), Ct stored [N][5].
@@ -69,8 +69,8 @@ __device__ __forceinline__ State EvalTaylor(const double *ct, const double *y) {
   double t[10];
   t[0] = 1.0; t[1] = y[0]; t[2] = y[1]; t[3] = y[2];
   if constexpr (P >= 2) {
-    t[4] = __dmul_rn(y[0], y[0]); t[5] = __dmul_rn(y[0], y[1]); t[6] = __dmul_rn(y[0], y[2]);
-    t[7] = __dmul_rn(y[1], y[1]); t[8] = __dmul_rn(y[1], y[2]); t[9] = __dmul_rn(y[2], y[2]);
+    t[4] = MulRn(y[0], y[0]); t[5] = MulRn(y[0], y[1]); t[6] = MulRn(y[0], y[2]);
+    t[7] = MulRn(y[1], y[1]); t[8] = MulRn(y[1], y[2]); t[9] = MulRn(y[2], y[2]);
   }
   State u;
 #pragma unroll
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
       const State u = EvalTaylor<P>(own_ct, y);
       double f[5][3];
       FluxMatrix(u, f);
-      const double w = __dmul_rn(wl, geo[9]);
+      const double w = MulRn(wl, geo[9]);
 #pragma unroll
       for (int k = 0; k < 5; ++k)
 #pragma unroll
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
       frame.b[0] = frame.n[1] * frame.t[2] - frame.n[2] * frame.t[1];
       frame.b[1] = frame.n[2] * frame.t[0] - frame.n[0] * frame.t[2];
       frame.b[2] = frame.n[0] * frame.t[1] - frame.n[1] * frame.t[0];
-      const double w = __dmul_rn(wl, fr[6]);
+      const double w = MulRn(wl, fr[6]);
       double y[3];
       Offset(geo, eta, y);
       const State own = EvalTaylor<P>(own_ct, y);
@@ -304,13 +304,13 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
     if (a.stage == 0) {
       out = res;
     } else {
-      const double next = __dadd_rn(us_pf[it], __dmul_rn(res, a.dt));
+      const double next = AddRn(us_pf[it], MulRn(res, a.dt));
       if (a.stage == 1) {
         out = next;
       } else if (a.stage == 2) {
-        out = __dadd_rn(__dmul_rn(u0_pf[it], 3.0), next) / 4.0;
+        out = AddRn(MulRn(u0_pf[it], 3.0), next) / 4.0;
       } else {
-        out = __dadd_rn(__dmul_rn(next, 2.0), u0_pf[it]) / 3.0;
+        out = AddRn(MulRn(next, 2.0), u0_pf[it]) / 3.0;
       }
     }
     a.out[idx] = out;
@@ -376,14 +376,14 @@ __global__ __launch_bounds__(256) void DgApproximateKernel(const double *values,
     double y[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d)
-      y[d] = __dadd_rn(__dadd_rn(__dmul_rn(e[0], g[d]), __dmul_rn(e[1], g[3 + d])), __dmul_rn(e[2], g[6 + d]));
-    const double t[10] = { 1.0, y[0], y[1], y[2], __dmul_rn(y[0], y[0]), __dmul_rn(y[0], y[1]),
-                           __dmul_rn(y[0], y[2]), __dmul_rn(y[1], y[1]), __dmul_rn(y[1], y[2]),
-                           __dmul_rn(y[2], y[2]) };
+      y[d] = AddRn(AddRn(MulRn(e[0], g[d]), MulRn(e[1], g[3 + d])), MulRn(e[2], g[6 + d]));
+    const double t[10] = { 1.0, y[0], y[1], y[2], MulRn(y[0], y[0]), MulRn(y[0], y[1]),
+                           MulRn(y[0], y[2]), MulRn(y[1], y[1]), MulRn(y[1], y[2]),
+                           MulRn(y[2], y[2]) };
     double phi = 0.0;
-    for (int j = 0; j <= i; ++j) phi = __dadd_rn(phi, __dmul_rn(s[j], t[j]));
-    const double prod = __dmul_rn(__dmul_rn(values[(c * nq + q) * 5 + k], phi), __dmul_rn(tb->vol_w[q], g[9]));
-    acc = __dadd_rn(acc, prod);
+    for (int j = 0; j <= i; ++j) phi = AddRn(phi, MulRn(s[j], t[j]));
+    const double prod = MulRn(MulRn(values[(c * nq + q) * 5 + k], phi), MulRn(tb->vol_w[q], g[9]));
+    acc = AddRn(acc, prod);
   }
   coeffs[w] = acc;
 }
@@ -438,8 +438,8 @@ __device__ __forceinline__ void EvalTaylorRhoE(const double *ct, const double *y
   double t[10];
   t[0] = 1.0; t[1] = y[0]; t[2] = y[1]; t[3] = y[2];
   if constexpr (P >= 2) {
-    t[4] = __dmul_rn(y[0], y[0]); t[5] = __dmul_rn(y[0], y[1]); t[6] = __dmul_rn(y[0], y[2]);
-    t[7] = __dmul_rn(y[1], y[1]); t[8] = __dmul_rn(y[1], y[2]); t[9] = __dmul_rn(y[2], y[2]);
+    t[4] = MulRn(y[0], y[0]); t[5] = MulRn(y[0], y[1]); t[6] = MulRn(y[0], y[2]);
+    t[7] = MulRn(y[1], y[1]); t[8] = MulRn(y[1], y[2]); t[9] = MulRn(y[2], y[2]);
   }
   double a0 = ct[0], a4 = ct[4];
 #pragma unroll

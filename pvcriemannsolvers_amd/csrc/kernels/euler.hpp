@@ -165,6 +165,22 @@ MCFD_HD State RoeFlux(State const &l, State const &r) {
   return flux;
 }
 
+// ------------------------------------------------- unfused rounding
+// One IEEE-rounded add or multiply that FP contraction may not merge into an
+// FMA with its neighbours: HIP's __dadd_rn / __dmul_rn are plain operators,
+// which the default contraction fuses (u + r * dt becomes one FMA).  Used
+// where the reference's operation order has to be kept to the last bit: the
+// RK combinations of temporal/rk.hpp:31-49 and ode.hpp:121-128, and the
+// host-order projection of Approximate.
+MCFD_HD double AddRn(double a, double b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+MCFD_HD double MulRn(double a, double b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+
 // ------------------------------------------------- fast FP64 reciprocal
 // v_rcp_f64 refined by two Newton steps: within 1 ulp of 1/x for normal x.
 // Used by the throughput kernels in place of IEEE division (a ~10

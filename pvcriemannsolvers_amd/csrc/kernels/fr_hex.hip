@@ -413,13 +413,13 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
         out = res;
       } else {
         // temporal/rk.hpp:31-49 with ode.hpp:121-128: residual *= dt; u += residual
-        const double next = __dadd_rn(us, __dmul_rn(res, dt));
+        const double next = AddRn(us, MulRn(res, dt));
         if (a.stage == 1) {
           out = next;
         } else if (a.stage == 2) {
-          out = __dadd_rn(__dmul_rn(u0v[k], 3.0), next) / 4.0;
+          out = AddRn(MulRn(u0v[k], 3.0), next) / 4.0;
         } else {
-          out = __dadd_rn(__dmul_rn(next, 2.0), u0v[k]) / 3.0;
+          out = AddRn(MulRn(next, 2.0), u0v[k]) / 3.0;
         }
       }
       outv[k] = out;
