@@ -52,7 +52,7 @@ struct Shape {
   static constexpr int kTasks = (kCells * kFaceTasks + Threads - 1) / Threads;  // per thread
   static_assert(kCells >= 1, "at most one workgroup of solution points per cell");
   // states, F~ columns / flux gaps, mixed reference values, face area scales
-  static constexpr int kLdsDoubles = kCells * 5 * kPad + kCells * kScratch + kCells * 5 + kCells * 6;
+  static constexpr int kLdsDoubles = kCells * 5 * kPad + kCells * kScratch + kCells * 15 + kCells * 6;
 };
 
 // Line of solution points through in-face point (a, b) of local face f:
@@ -159,8 +159,8 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
   constexpr int PAD = S::kPad, FPAD = S::kFacePad, T = S::kTasks;
   double *s_u = lds;                          // states, SoA per cell
   double *s_x = lds + C * 5 * PAD;            // F~ column per pass, then flux gaps
-  double *s_ref = s_x + C * S::kScratch;      // mixed: FP64 column value at node 0
-  double *s_scale = s_ref + C * 5;            // signed face area scales of each cell
+  double *s_ref = s_x + C * S::kScratch;      // mixed: FP64 column values at node 0, [d][cell][5]
+  double *s_scale = s_ref + C * 15;           // signed face area scales of each cell
   float *s_x32 = reinterpret_cast<float *>(s_x);
 
   const int tid = threadIdx.x;
@@ -225,6 +225,20 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
   }
   double fn[T][5];
 
+  // mixed: node 0 of each cell publishes its three F~ columns, the FP64
+  // references of the FP32 deviations; the first pass's barrier orders them
+  if constexpr (kMixed) {
+    if (node_active && node == 0) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double col[5];
+        LocalFluxColumn<kDiag>(cu, pu, js, d, col);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) s_ref[(d * C + ncell) * 5 + k] = col[k];
+      }
+    }
+  }
+
   // ---- three passes: one column of F~ at a time through LDS
   double div[5] = { 0, 0, 0, 0, 0 };
 #pragma unroll
@@ -234,21 +248,12 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
     if (node_active) {
       LocalFluxColumn<kDiag>(cu, pu, js, d, col);
       if constexpr (kMixed) {
-        if (node == 0) {
 #pragma unroll
-          for (int k = 0; k < 5; ++k) s_ref[ncell * 5 + k] = col[k];
-        }
+        for (int k = 0; k < 5; ++k)
+          s_x32[ncell * S::kScratch + k * PAD + spos] = (float)(col[k] - s_ref[(d * C + ncell) * 5 + k]);
       } else {
 #pragma unroll
         for (int k = 0; k < 5; ++k) s_x[ncell * S::kScratch + k * PAD + spos] = col[k];
-      }
-    }
-    if constexpr (kMixed) {
-      __syncthreads();
-      if (node_active) {
-#pragma unroll
-        for (int k = 0; k < 5; ++k)
-          s_x32[ncell * S::kScratch + k * PAD + spos] = (float)(col[k] - s_ref[ncell * 5 + k]);
       }
     }
     __syncthreads();
@@ -289,7 +294,7 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
             float acc = (float)lw[0] * sx[k * PAD + l0];
 #pragma unroll
             for (int m = 1; m < Q; ++m) acc = __fmaf_rn((float)lw[m], sx[k * PAD + l0 + m * li.stride], acc);
-            fn[t][k] = s_ref[t_cell[t] * 5 + k] + (double)acc;
+            fn[t][k] = s_ref[(d * C + t_cell[t]) * 5 + k] + (double)acc;
           }
         } else {
           const double *sx = s_x + t_cell[t] * S::kScratch;
