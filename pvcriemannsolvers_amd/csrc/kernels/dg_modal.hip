@@ -96,8 +96,10 @@ __device__ __forceinline__ MonomialCode MonomialOf(int j) {
   return m;
 }
 
+// The exact solver's iteration needs more registers than 4 waves per SIMD
+// allow; its instantiations trade occupancy for no spills.
 template <bool kHex, int P, int kRiemann>
-__global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
+__global__ __launch_bounds__(256, kRiemann == kExact ? 2 : 4) void DgModalStageKernel(DgModalArgs a) {
   using S = DgShape<kHex, P>;
   constexpr int N = S::kN, NQ = S::kNQ, NF = S::kNF, NFP = S::kNFP, T = S::kT, G = S::kG, PK = S::kPacked;
   constexpr int NB = 5 * N + 10;  // neighbour record: Taylor coefficients, J rows, pad
