@@ -294,3 +294,15 @@ def test_partial_tile_and_largest_degree(degree, riemann):
         g.rk3_update(s * dt, dt)
     assert rel(g.get_solution_column(), ref) < 1e-12
     g.close()
+
+
+@pytest.mark.parametrize("degree", [1, 3, 4])
+def test_lumped_lobatto_matches_oracle(degree):
+    """fr::Lobatto (spatial/fr/lobatto.hpp) is fr::General on Lobatto nodes with Huynh's lumping
+    c_next = p / (2p + 1) (basis/vincent.hpp:31-33; the reference's test/spatial/fr.cpp:82-97)."""
+    mesh = meshgen.box_hex(3, 4, 3, periodic=(True, True, True), shuffle_seed=degree)
+    o, g = make_pair(mesh, degree, "roe", nodes="lobatto", c_next=degree / (2 * degree + 1))
+    u = density_wave(o.node_coords(), seed=degree)
+    g.set_solution_column(u.reshape(-1))
+    assert rel(g.get_residual_column(), o.residual(u).reshape(-1)) < 1e-12
+    g.close()
