@@ -323,6 +323,7 @@ void DgGroup::Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &scheme,
 
   // ---- face records and per-cell face info
   h_faces.assign(faces.size() * kDgFaceStride, 0.0);
+  h_face_cells.assign(faces.size() * 3, 0);
   h_nbr.assign(n_cells * 6, INT32_MIN);
   h_finfo.assign(n_cells * 6, 0);
   for (size_t f = 0; f < faces.size(); ++f) {
@@ -349,6 +350,9 @@ void DgGroup::Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &scheme,
     const double g11 = Dot(ra, ra), g12 = Dot(ra, rb), g22 = Dot(rb, rb);
     fr[6] = std::sqrt(g11 * g22 - g12 * g12);  // integrator/element.hpp:108-110
     const int32_t fid = (int32_t)f << 8;
+    h_face_cells[f * 3] = (int32_t)face.holder;
+    h_face_cells[f * 3 + 1] = face.sharer >= 0 ? (int32_t)face.sharer : -1 - face.tag;
+    h_face_cells[f * 3 + 2] = face.holder_face | (face.sharer >= 0 ? face.sharer_face : 0) << 3 | face.orient << 6;
     if (face.sharer >= 0) {
       h_nbr[face.holder * 6 + face.holder_face] = (int32_t)face.sharer;
       h_finfo[face.holder * 6 + face.holder_face] = fid | 1 << 7 | face.orient << 3 | face.sharer_face;
@@ -387,6 +391,10 @@ void DgGroup::Upload(hipStream_t stream) {
   basis.Upload(h_basis, stream);
   geo.Upload(h_geo, stream);
   face_rec.Upload(h_faces, stream);
+  if (split_faces) {
+    face_cells.Upload(h_face_cells, stream);
+    fhat.Allocate(faces.size() * nfp * 5);
+  }
   bc_state.Upload(h_bc_state, stream);
   nbr.Upload(h_nbr, stream);
   finfo.Upload(h_finfo, stream);
@@ -436,6 +444,9 @@ void DgGroup::Stage(int stage, double dt, hipStream_t stream, unsigned long long
   a.bc_kind = bc_kind.ptr; a.bc_state = bc_state.ptr; a.tables = tables_d.ptr;
   a.n_cells = n_cells; a.dt = dt; a.stage = stage;
   a.dt_min_bits = stage > 0 && !limiter ? dt_min_bits : nullptr;
+  a.fhat = split_faces ? fhat.ptr : nullptr;
+  a.face_cells = face_cells.ptr;
+  a.n_faces = (int64_t)faces.size();
   Check(LaunchDgModalStage(cell_type, degree, riemann, a, stream), "DG stage launch");
   if (stage > 0) Limit(a.out, a.ct_out, stream);  // spatial/with_limiter.hpp:56-59 after each SetSolutionColumn
 }

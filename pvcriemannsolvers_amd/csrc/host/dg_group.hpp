@@ -34,6 +34,11 @@ struct DgGroup {
   DeviceArray<double> u0, u1, u2, res, basis, geo, face_rec, bc_state;
   DeviceArray<double> ct0, ct1, ct2;  // Taylor coefficients of u0, u1, u2 (kept current)
   DeviceArray<int32_t> nbr, finfo, bc_kind;
+  // face-split stage: w Fhat once per face point, then the cell pass
+  bool split_faces = true;
+  std::vector<int32_t> h_face_cells;  // [face][3]: holder, sharer or -(1 + tag), faces and orientation
+  DeviceArray<int32_t> face_cells;
+  DeviceArray<double> fhat;           // [face][point][5]
   DeviceArray<DgTables> tables_d;
   // WENO limiter (MCFD_LIMITER_*), applied to every state that is set
   int limiter = 0;

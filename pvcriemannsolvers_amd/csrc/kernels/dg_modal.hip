@@ -97,18 +97,20 @@ __device__ __forceinline__ MonomialCode MonomialOf(int j) {
 }
 
 // The exact solver's iteration needs more registers than 4 waves per SIMD
-// allow; its instantiations trade occupancy for no spills.
-template <bool kHex, int P, int kRiemann>
+// allow; its instantiations trade occupancy for no spills.  kSplit: the face
+// fluxes w Fhat were computed once per face point by DgFaceFluxKernel into
+// a.fhat, and the face tasks only read them (no neighbour staging).
+template <bool kHex, int P, int kRiemann, bool kSplit>
 __global__ __launch_bounds__(256, kRiemann == kExact ? 2 : 4) void DgModalStageKernel(DgModalArgs a) {
   using S = DgShape<kHex, P>;
   constexpr int N = S::kN, NQ = S::kNQ, NF = S::kNF, NFP = S::kNFP, T = S::kT, G = S::kG, PK = S::kPacked;
   constexpr int NB = 5 * N + 10;  // neighbour record: Taylor coefficients, J rows, pad
   constexpr int GS = kDgGeoStride;
   __shared__ double s_own[G * 5 * N + G * GS];  // own Taylor coefficients and geometry; then S
-  __shared__ double s_nbr[G * NF * NB];    // neighbours' coefficients and J; then the moments Rt
+  __shared__ double s_nbr[kSplit ? G * 5 * N : G * NF * NB];  // neighbours' coefficients and J; then Rt
   __shared__ double s_vol[G * NQ * kSlot]; // volume tasks: flux x weight, zero column, Y
   __shared__ double s_face[G * NF * NFP * kFaceSlot];  // face tasks: -/+ w Fhat (5), Y; then new coefficients
-  static_assert(G * 5 * N <= G * NF * NB && G * 5 * N <= G * NF * NFP * kFaceSlot, "LDS aliasing");
+  static_assert(G * 5 * N <= G * NF * NFP * kFaceSlot, "LDS aliasing");
   static_assert(G * PK <= G * 5 * N + G * GS, "LDS aliasing");
   static_assert(G * N + G <= G * NQ * kSlot, "LDS aliasing");
   double *s_ct = s_own, *s_geo = s_own + G * 5 * N;
@@ -135,11 +137,21 @@ __global__ __launch_bounds__(256, kRiemann == kExact ? 2 : 4) void DgModalStageK
   const DgTables *tb = a.tables;
   int32_t nb = 0, info = 0;
   double eta[3] = { 0, 0, 0 }, eta_nb[3] = { 0, 0, 0 }, wl = 0, fr[7] = { 0, 0, 0, 0, 0, 0, 0 };
+  double fw[5] = { 0, 0, 0, 0, 0 };  // kSplit: w Fhat of this face point
   if (active) {
     if (!is_face) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) eta[d] = tb->vol_eta[r][d];
       wl = tb->vol_w[r];
+    } else if constexpr (kSplit) {
+      info = a.finfo[(c0 + g) * 6 + lf];
+      const bool holder = (info >> 7) & 1;
+      const int orient = (info >> 3) & 15;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) eta[d] = tb->face_eta[lf][holder ? 0 : orient][q][d];
+      const double *src = a.fhat + ((int64_t)(info >> 8) * NFP + q) * 5;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) fw[k] = src[k];
     } else {
       nb = a.nbr[(c0 + g) * 6 + lf];
       info = a.finfo[(c0 + g) * 6 + lf];
@@ -161,8 +173,8 @@ __global__ __launch_bounds__(256, kRiemann == kExact ? 2 : 4) void DgModalStageK
   // ---- phase 0: stage the cells' and their neighbours' data in LDS
   for (int w = tid; w < nc * 5 * N; w += 256) s_ct[w] = a.ct[c0 * 5 * N + w];
   for (int w = tid; w < nc * GS; w += 256) s_geo[w] = a.geo[c0 * GS + w];
-  constexpr int kNbIter = (G * NF * NB + 255) / 256;
-  int32_t nbn[kNbIter];
+  constexpr int kNbIter = kSplit ? 0 : (G * NF * NB + 255) / 256;
+  int32_t nbn[kNbIter > 0 ? kNbIter : 1];
 #pragma unroll
   for (int it = 0; it < kNbIter; ++it) {
     const int w = tid + 256 * it;
@@ -198,6 +210,17 @@ __global__ __launch_bounds__(256, kRiemann == kExact ? 2 : 4) void DgModalStageK
 #pragma unroll
       for (int d = 0; d < 3; ++d) slot[kSlotY + d] = y[d];
       slot[kSlotY + 3] = 1.0;
+    } else if constexpr (kSplit) {
+      // dg/general.hpp:80-88: -w Fhat into the holder, +w Fhat into the sharer
+      double *slot = s_face + (g * NF * NFP + (r - NQ)) * kFaceSlot;
+      const bool holder = (info >> 7) & 1;
+      double y[3];
+      Offset(geo, eta, y);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) slot[k] = holder ? -fw[k] : fw[k];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) slot[5 + d] = y[d];
+      slot[8] = 1.0;
     } else {
       double *slot = s_face + (g * NF * NFP + (r - NQ)) * kFaceSlot;
       const bool holder = (info >> 7) & 1;
@@ -355,6 +378,59 @@ __global__ __launch_bounds__(256, kRiemann == kExact ? 2 : 4) void DgModalStageK
     for (int c = 0; c < nc; ++c) local = fmin(local, s_cell[c]);
     atomicMin(a.dt_min_bits, (unsigned long long)__double_as_longlong(local));
   }
+}
+
+// w Fhat at every face quadrature point, once per face (dg/general.hpp:51-55,
+// 70-89 and the boundary fluxes :92-202): one thread per (face, point).  The
+// holder's and the sharer's traces are evaluated from their Taylor
+// coefficients with the same table entries and FMA chains as the face tasks
+// of the unsplit stage kernel, so the fluxes are bitwise the ones that kernel
+// computes on either side.
+template <bool kHex, int P, int kRiemann>
+__global__ __launch_bounds__(256) void DgFaceFluxKernel(DgModalArgs a) {
+  constexpr int N = P == 1 ? 4 : 10, NFP = DgShape<kHex, P>::kNFP, GS = kDgGeoStride;
+  const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (w >= a.n_faces * NFP) return;
+  const int64_t f = w / NFP;
+  const int q = (int)(w - f * NFP);
+  const DgTables *tb = a.tables;
+  const int32_t holder = a.face_cells[f * 3], sharer = a.face_cells[f * 3 + 1], pk = a.face_cells[f * 3 + 2];
+  const int hf = pk & 7, sf = (pk >> 3) & 7, orient = (pk >> 6) & 15;
+  const double *fc = a.faces + f * kDgFaceStride;
+  FaceFrame frame;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) { frame.n[d] = fc[d]; frame.t[d] = fc[3 + d]; }
+  frame.b[0] = frame.n[1] * frame.t[2] - frame.n[2] * frame.t[1];
+  frame.b[1] = frame.n[2] * frame.t[0] - frame.n[0] * frame.t[2];
+  frame.b[2] = frame.n[0] * frame.t[1] - frame.n[1] * frame.t[0];
+  const double wq = MulRn(tb->face_w[q], fc[6]);
+  double y[3];
+  Offset(a.geo + (int64_t)holder * GS, tb->face_eta[hf][0][q], y);
+  const State own = EvalTaylor<P>(a.ct + (int64_t)holder * 5 * N, y);
+  State flux;
+  if (sharer >= 0) {
+    double ys[3];
+    Offset(a.geo + (int64_t)sharer * GS, tb->face_eta[sf][orient][q], ys);
+    const State other = EvalTaylor<P>(a.ct + (int64_t)sharer * 5 * N, ys);
+    flux = UpwindFlux<kRiemann>(frame, own, other);
+  } else {
+    const int tag = -sharer - 1;
+    const int kind = a.bc_kind[tag];
+    State given;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) given.v[k] = a.bc_state[tag * 5 + k];
+    if (kind == 1) {
+      flux = WallFlux(frame, own);
+    } else if (kind == 2) {
+      flux = StateFlux(frame, own);
+    } else if (kind == 3) {
+      flux = StateFlux(frame, given);
+    } else {
+      flux = UpwindFlux<kRiemann>(frame, own, given);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) a.fhat[w * 5 + k] = flux.v[k] * wq;
 }
 
 // projection.hpp:28-43 (Project) of point values f at the cell quadrature
@@ -852,7 +928,16 @@ hipError_t LaunchStage(const DgModalArgs &a, hipStream_t stream) {
   constexpr int G = DgShape<kHex, P>::kG;
   const int64_t blocks = (a.n_cells + G - 1) / G;
   if (blocks > 0x7fffffff) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((DgModalStageKernel<kHex, P, R>), dim3((unsigned)blocks), dim3(256), 0, stream, a);
+  if (a.fhat) {  // face fluxes once per face point, then the cell pass
+    constexpr int NFP = DgShape<kHex, P>::kNFP;
+    const int64_t fblocks = (a.n_faces * NFP + 255) / 256;
+    if (fblocks > 0x7fffffff) return hipErrorInvalidValue;
+    if (fblocks > 0)
+      hipLaunchKernelGGL((DgFaceFluxKernel<kHex, P, R>), dim3((unsigned)fblocks), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL((DgModalStageKernel<kHex, P, R, true>), dim3((unsigned)blocks), dim3(256), 0, stream, a);
+  } else {
+    hipLaunchKernelGGL((DgModalStageKernel<kHex, P, R, false>), dim3((unsigned)blocks), dim3(256), 0, stream, a);
+  }
   return hipGetLastError();
 }
 

@@ -55,6 +55,13 @@ struct DgModalArgs {
   // If set (with ct_out), the time-step reduction of LaunchDgTimeStep over the
   // state this pass writes is folded into *dt_min_bits (same arithmetic).
   unsigned long long *dt_min_bits;
+  // If set, the pass runs as two launches: w Fhat once per face point into
+  // fhat [face][point][5], then the cells read it.  face_cells [face][3]:
+  // holder, sharer or -(1 + boundary tag), holder face | sharer face << 3 |
+  // orientation << 6.
+  double *fhat;
+  const int32_t *face_cells;
+  int64_t n_faces;
 };
 
 // Ct = C S for every cell of `us` into `ct`.
