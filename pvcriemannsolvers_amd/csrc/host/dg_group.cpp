@@ -391,12 +391,9 @@ void DgGroup::Upload(hipStream_t stream) {
   basis.Upload(h_basis, stream);
   geo.Upload(h_geo, stream);
   face_rec.Upload(h_faces, stream);
-  if (split_faces) {
-    face_cells.Upload(h_face_cells, stream);
-    fhat.Allocate(faces.size() * nfp * 5);
-  }
+  face_cells.Upload(h_face_cells, stream);
+  fhat.Allocate(faces.size() * nfp * 5);
   bc_state.Upload(h_bc_state, stream);
-  nbr.Upload(h_nbr, stream);
   finfo.Upload(h_finfo, stream);
   bc_kind.Upload(h_bc_kind, stream);
   h_tables.assign(1, tables);
@@ -440,11 +437,11 @@ void DgGroup::Stage(int stage, double dt, hipStream_t stream, unsigned long long
   default: a.us = u2.ptr; a.ct = ct2.ptr; a.out = u0.ptr; a.ct_out = ct0.ptr; break;
   }
   a.basis = basis.ptr; a.geo = geo.ptr;
-  a.nbr = nbr.ptr; a.finfo = finfo.ptr; a.faces = face_rec.ptr;
+  a.finfo = finfo.ptr; a.faces = face_rec.ptr;
   a.bc_kind = bc_kind.ptr; a.bc_state = bc_state.ptr; a.tables = tables_d.ptr;
   a.n_cells = n_cells; a.dt = dt; a.stage = stage;
   a.dt_min_bits = stage > 0 && !limiter ? dt_min_bits : nullptr;
-  a.fhat = split_faces ? fhat.ptr : nullptr;
+  a.fhat = fhat.ptr;
   a.face_cells = face_cells.ptr;
   a.n_faces = (int64_t)faces.size();
   Check(LaunchDgModalStage(cell_type, degree, riemann, a, stream), "DG stage launch");

@@ -33,9 +33,8 @@ struct DgGroup {
   std::vector<double> qp;              // global quadrature points [cell][q][3]
   DeviceArray<double> u0, u1, u2, res, basis, geo, face_rec, bc_state;
   DeviceArray<double> ct0, ct1, ct2;  // Taylor coefficients of u0, u1, u2 (kept current)
-  DeviceArray<int32_t> nbr, finfo, bc_kind;
-  // face-split stage: w Fhat once per face point, then the cell pass
-  bool split_faces = true;
+  DeviceArray<int32_t> finfo, bc_kind;
+  // the stage's face pass: w Fhat once per face point, then the cell pass
   std::vector<int32_t> h_face_cells;  // [face][3]: holder, sharer or -(1 + tag), faces and orientation
   DeviceArray<int32_t> face_cells;
   DeviceArray<double> fhat;           // [face][point][5]

@@ -1,27 +1,29 @@
 // Modal DG stage kernels for affine hexahedra and tetrahedra (gfx950).
 //
-// Decomposition of one stage pass: a workgroup of 256 threads owns G cells;
-// each cell has T = NQ + NF * NFP point tasks (its volume quadrature points
-// and the quadrature points of its faces), one thread per task, volume tasks
-// of all G cells first.
-//   phase 0  the cells' Taylor coefficients and geometry, and each
-//            neighbour's coefficients and Jacobian, are staged in LDS (one
-//            cooperative load per face, not per point);
+// One stage pass is two launches.
+//
+// DgFaceFluxKernel, one thread per face quadrature point: the holder's and
+// the sharer's traces from their Taylor coefficients, the Riemann or boundary
+// flux, times the weight -> fhat [face][point][5].  Each face is evaluated
+// once, so its two cells read the same flux and the scheme stays
+// conservative without atomics.
+//
+// DgModalStageKernel, a workgroup of 256 threads owning G cells; each cell
+// has T = NQ + NF * NFP point tasks (its volume quadrature points and the
+// quadrature points of its faces), one thread per task, volume tasks of all
+// G cells first.
+//   phase 0  the cells' Taylor coefficients and geometry are staged in LDS;
 //   phase 1  volume task: state u = Ct T(y) at the point, flux matrix times
-//            the global weight -> LDS;  face task: own trace, neighbour trace
-//            (from the neighbour's Taylor coefficients and Jacobian),
-//            Riemann or boundary flux times -/+ the weight -> LDS;
+//            the global weight -> LDS;  face task: -/+ w Fhat read from fhat
+//            and the point's offset Y -> LDS;
 //   phase 2  one thread per (cell, component, Taylor monomial) sums the
 //            moments Rt over the cell's tasks, while the Gram-Schmidt factors
 //            and the state / U0 of phase 3 arrive;
 //   phase 3  one thread per (cell, basis function, component): R = Rt S^T,
 //            the SSP-RK3 combination (temporal/rk.hpp:31-49 operation order)
 //            and the store;
-//   phase 4  the new state's Taylor coefficients for the next pass.
-// An interior face is evaluated by both of its cells from identical inputs
-// (the same table entries, Jacobians and coefficients through the same
-// fixed-order FMA chains), so the two sides see bitwise the same flux and
-// the scheme stays conservative without atomics.
+//   phase 4  the new state's Taylor coefficients for the next pass;
+//   phase 5  (optional) the time-step reduction of the new state.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -96,18 +98,15 @@ __device__ __forceinline__ MonomialCode MonomialOf(int j) {
   return m;
 }
 
-// The exact solver's iteration needs more registers than 4 waves per SIMD
-// allow; its instantiations trade occupancy for no spills.  kSplit: the face
-// fluxes w Fhat were computed once per face point by DgFaceFluxKernel into
-// a.fhat, and the face tasks only read them (no neighbour staging).
-template <bool kHex, int P, int kRiemann, bool kSplit>
-__global__ __launch_bounds__(256, kRiemann == kExact ? 2 : 4) void DgModalStageKernel(DgModalArgs a) {
+// The face fluxes w Fhat were computed once per face point by
+// DgFaceFluxKernel into a.fhat; the face tasks read them.
+template <bool kHex, int P>
+__global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
   using S = DgShape<kHex, P>;
   constexpr int N = S::kN, NQ = S::kNQ, NF = S::kNF, NFP = S::kNFP, T = S::kT, G = S::kG, PK = S::kPacked;
-  constexpr int NB = 5 * N + 10;  // neighbour record: Taylor coefficients, J rows, pad
   constexpr int GS = kDgGeoStride;
   __shared__ double s_own[G * 5 * N + G * GS];  // own Taylor coefficients and geometry; then S
-  __shared__ double s_nbr[kSplit ? G * 5 * N : G * NF * NB];  // neighbours' coefficients and J; then Rt
+  __shared__ double s_rt[G * 5 * N];       // Taylor moments Rt
   __shared__ double s_vol[G * NQ * kSlot]; // volume tasks: flux x weight, zero column, Y
   __shared__ double s_face[G * NF * NFP * kFaceSlot];  // face tasks: -/+ w Fhat (5), Y; then new coefficients
   static_assert(G * 5 * N <= G * NF * NFP * kFaceSlot, "LDS aliasing");
@@ -115,7 +114,6 @@ __global__ __launch_bounds__(256, kRiemann == kExact ? 2 : 4) void DgModalStageK
   static_assert(G * N + G <= G * NQ * kSlot, "LDS aliasing");
   double *s_ct = s_own, *s_geo = s_own + G * 5 * N;
   double *s_s = s_own;  // Gram-Schmidt factors, from phase 2 on
-  double *s_rt = s_nbr;
   double *s_new = s_face;
   constexpr int kSIter = (G * PK + 255) / 256, kItems = (G * 5 * N + 255) / 256;
 
@@ -135,15 +133,15 @@ __global__ __launch_bounds__(256, kRiemann == kExact ? 2 : 4) void DgModalStageK
   const bool active = g < nc && tid < G * T;
   const int lf = is_face ? (r - NQ) / NFP : 0, q = is_face ? (r - NQ) - ((r - NQ) / NFP) * NFP : r;
   const DgTables *tb = a.tables;
-  int32_t nb = 0, info = 0;
-  double eta[3] = { 0, 0, 0 }, eta_nb[3] = { 0, 0, 0 }, wl = 0, fr[7] = { 0, 0, 0, 0, 0, 0, 0 };
-  double fw[5] = { 0, 0, 0, 0, 0 };  // kSplit: w Fhat of this face point
+  int32_t info = 0;
+  double eta[3] = { 0, 0, 0 }, wl = 0;
+  double fw[5] = { 0, 0, 0, 0, 0 };  // face task: w Fhat of this face point
   if (active) {
     if (!is_face) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) eta[d] = tb->vol_eta[r][d];
       wl = tb->vol_w[r];
-    } else if constexpr (kSplit) {
+    } else {
       info = a.finfo[(c0 + g) * 6 + lf];
       const bool holder = (info >> 7) & 1;
       const int orient = (info >> 3) & 15;
@@ -152,42 +150,12 @@ __global__ __launch_bounds__(256, kRiemann == kExact ? 2 : 4) void DgModalStageK
       const double *src = a.fhat + ((int64_t)(info >> 8) * NFP + q) * 5;
 #pragma unroll
       for (int k = 0; k < 5; ++k) fw[k] = src[k];
-    } else {
-      nb = a.nbr[(c0 + g) * 6 + lf];
-      info = a.finfo[(c0 + g) * 6 + lf];
-      const bool holder = (info >> 7) & 1;
-      const int orient = (info >> 3) & 15, nf = info & 7;
-#pragma unroll
-      for (int d = 0; d < 3; ++d) eta[d] = tb->face_eta[lf][holder ? 0 : orient][q][d];
-      if (nb >= 0) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) eta_nb[d] = tb->face_eta[nf][holder ? orient : 0][q][d];
-      }
-      wl = tb->face_w[q];
-      const double *fc = a.faces + (int64_t)(info >> 8) * kDgFaceStride;
-#pragma unroll
-      for (int i = 0; i < 7; ++i) fr[i] = fc[i];
     }
   }
 
-  // ---- phase 0: stage the cells' and their neighbours' data in LDS
+  // ---- phase 0: stage the cells' data in LDS
   for (int w = tid; w < nc * 5 * N; w += 256) s_ct[w] = a.ct[c0 * 5 * N + w];
   for (int w = tid; w < nc * GS; w += 256) s_geo[w] = a.geo[c0 * GS + w];
-  constexpr int kNbIter = kSplit ? 0 : (G * NF * NB + 255) / 256;
-  int32_t nbn[kNbIter > 0 ? kNbIter : 1];
-#pragma unroll
-  for (int it = 0; it < kNbIter; ++it) {
-    const int w = tid + 256 * it;
-    const int gl = w / NB, gg = gl / NF, lfn = gl - (gl / NF) * NF;
-    nbn[it] = w < nc * NF * NB ? a.nbr[(c0 + gg) * 6 + lfn] : -1;
-  }
-#pragma unroll
-  for (int it = 0; it < kNbIter; ++it) {
-    const int w = tid + 256 * it;
-    const int e = w - (w / NB) * NB;
-    if (nbn[it] >= 0 && e < 5 * N + 9)
-      s_nbr[w] = e < 5 * N ? a.ct[(int64_t)nbn[it] * 5 * N + e] : a.geo[(int64_t)nbn[it] * GS + (e - 5 * N)];
-  }
   __syncthreads();
 
   // ---- phase 1: point tasks
@@ -210,7 +178,7 @@ __global__ __launch_bounds__(256, kRiemann == kExact ? 2 : 4) void DgModalStageK
 #pragma unroll
       for (int d = 0; d < 3; ++d) slot[kSlotY + d] = y[d];
       slot[kSlotY + 3] = 1.0;
-    } else if constexpr (kSplit) {
+    } else {
       // dg/general.hpp:80-88: -w Fhat into the holder, +w Fhat into the sharer
       double *slot = s_face + (g * NF * NFP + (r - NQ)) * kFaceSlot;
       const bool holder = (info >> 7) & 1;
@@ -218,50 +186,6 @@ __global__ __launch_bounds__(256, kRiemann == kExact ? 2 : 4) void DgModalStageK
       Offset(geo, eta, y);
 #pragma unroll
       for (int k = 0; k < 5; ++k) slot[k] = holder ? -fw[k] : fw[k];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) slot[5 + d] = y[d];
-      slot[8] = 1.0;
-    } else {
-      double *slot = s_face + (g * NF * NFP + (r - NQ)) * kFaceSlot;
-      const bool holder = (info >> 7) & 1;
-      FaceFrame frame;
-#pragma unroll
-      for (int d = 0; d < 3; ++d) { frame.n[d] = fr[d]; frame.t[d] = fr[3 + d]; }
-      frame.b[0] = frame.n[1] * frame.t[2] - frame.n[2] * frame.t[1];
-      frame.b[1] = frame.n[2] * frame.t[0] - frame.n[0] * frame.t[2];
-      frame.b[2] = frame.n[0] * frame.t[1] - frame.n[1] * frame.t[0];
-      const double w = MulRn(wl, fr[6]);
-      double y[3];
-      Offset(geo, eta, y);
-      const State own = EvalTaylor<P>(own_ct, y);
-      State flux;
-      double sgn = -1.0;
-      if (nb >= 0) {
-        const double *rec = s_nbr + (g * NF + lf) * NB;
-        double yn[3];
-        Offset(rec + 5 * N, eta_nb, yn);
-        const State other = EvalTaylor<P>(rec, yn);
-        flux = UpwindFlux<kRiemann>(frame, holder ? own : other, holder ? other : own);
-        sgn = holder ? -1.0 : 1.0;
-      } else {
-        const int tag = -nb - 1;
-        const int kind = a.bc_kind[tag];
-        State given;
-#pragma unroll
-        for (int k = 0; k < 5; ++k) given.v[k] = a.bc_state[tag * 5 + k];
-        if (kind == 1) {
-          flux = WallFlux(frame, own);
-        } else if (kind == 2) {
-          flux = StateFlux(frame, own);
-        } else if (kind == 3) {
-          flux = StateFlux(frame, given);
-        } else {
-          flux = UpwindFlux<kRiemann>(frame, own, given);
-        }
-      }
-      // dg/general.hpp:80-88: flux *= -w into the holder, -(flux * -w) into the sharer
-#pragma unroll
-      for (int k = 0; k < 5; ++k) slot[k] = flux.v[k] * (sgn * w);
 #pragma unroll
       for (int d = 0; d < 3; ++d) slot[5 + d] = y[d];
       slot[8] = 1.0;
@@ -382,10 +306,11 @@ __global__ __launch_bounds__(256, kRiemann == kExact ? 2 : 4) void DgModalStageK
 
 // w Fhat at every face quadrature point, once per face (dg/general.hpp:51-55,
 // 70-89 and the boundary fluxes :92-202): one thread per (face, point).  The
-// holder's and the sharer's traces are evaluated from their Taylor
-// coefficients with the same table entries and FMA chains as the face tasks
-// of the unsplit stage kernel, so the fluxes are bitwise the ones that kernel
-// computes on either side.
+// point is located in the holder through its canonical order and in the
+// sharer through the corner-permutation code, and each trace is evaluated
+// from the cell's Taylor coefficients.  The exact solver's iteration needs
+// more registers than 4 waves per SIMD allow; the default bound leaves the
+// compiler room for it without spills.
 template <bool kHex, int P, int kRiemann>
 __global__ __launch_bounds__(256) void DgFaceFluxKernel(DgModalArgs a) {
   constexpr int N = P == 1 ? 4 : 10, NFP = DgShape<kHex, P>::kNFP, GS = kDgGeoStride;
@@ -928,16 +853,13 @@ hipError_t LaunchStage(const DgModalArgs &a, hipStream_t stream) {
   constexpr int G = DgShape<kHex, P>::kG;
   const int64_t blocks = (a.n_cells + G - 1) / G;
   if (blocks > 0x7fffffff) return hipErrorInvalidValue;
-  if (a.fhat) {  // face fluxes once per face point, then the cell pass
-    constexpr int NFP = DgShape<kHex, P>::kNFP;
-    const int64_t fblocks = (a.n_faces * NFP + 255) / 256;
-    if (fblocks > 0x7fffffff) return hipErrorInvalidValue;
-    if (fblocks > 0)
-      hipLaunchKernelGGL((DgFaceFluxKernel<kHex, P, R>), dim3((unsigned)fblocks), dim3(256), 0, stream, a);
-    hipLaunchKernelGGL((DgModalStageKernel<kHex, P, R, true>), dim3((unsigned)blocks), dim3(256), 0, stream, a);
-  } else {
-    hipLaunchKernelGGL((DgModalStageKernel<kHex, P, R, false>), dim3((unsigned)blocks), dim3(256), 0, stream, a);
-  }
+  constexpr int NFP = DgShape<kHex, P>::kNFP;
+  const int64_t fblocks = (a.n_faces * NFP + 255) / 256;
+  if (fblocks > 0x7fffffff || !a.fhat || !a.face_cells) return hipErrorInvalidValue;
+  // face fluxes once per face point, then the cell pass
+  if (fblocks > 0)
+    hipLaunchKernelGGL((DgFaceFluxKernel<kHex, P, R>), dim3((unsigned)fblocks), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL((DgModalStageKernel<kHex, P>), dim3((unsigned)blocks), dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 

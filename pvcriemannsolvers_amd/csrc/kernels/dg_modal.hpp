@@ -43,7 +43,6 @@ struct DgModalArgs {
   double *ct_out;         // Taylor coefficients of `out`, or null
   const double *basis;    // [cell][N(N+1)/2] S, packed lower triangle by rows
   const double *geo;      // [cell][kDgGeoStride]
-  const int32_t *nbr;     // [cell][6]: neighbour cell, or -(1 + boundary tag)
   const int32_t *finfo;   // [cell][6]: face id << 8 | holder << 7 | orient << 3 | neighbour's local face
   const double *faces;    // [face][kDgFaceStride]
   const int32_t *bc_kind; // [tag]: 0 smart, 1 inviscid wall, 2 supersonic outlet, 3 inlet
@@ -55,9 +54,9 @@ struct DgModalArgs {
   // If set (with ct_out), the time-step reduction of LaunchDgTimeStep over the
   // state this pass writes is folded into *dt_min_bits (same arithmetic).
   unsigned long long *dt_min_bits;
-  // If set, the pass runs as two launches: w Fhat once per face point into
-  // fhat [face][point][5], then the cells read it.  face_cells [face][3]:
-  // holder, sharer or -(1 + boundary tag), holder face | sharer face << 3 |
+  // The pass runs as two launches: w Fhat once per face point into fhat
+  // [face][point][5], then the cells read it.  face_cells [face][3]: holder,
+  // sharer or -(1 + boundary tag), holder face | sharer face << 3 |
   // orientation << 6.
   double *fhat;
   const int32_t *face_cells;
