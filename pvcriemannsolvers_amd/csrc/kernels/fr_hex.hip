@@ -8,10 +8,11 @@
 // pass
 //   0. loads the cells' states (one thread per solution point) and keeps a
 //      skewed structure-of-arrays copy in LDS;
-//   1. three passes, one per reference direction: the F~ = F J* column goes
-//      through LDS, every solution point accumulates its sum-factorised
-//      Q-point derivative and every flux point normal to that direction
-//      extrapolates the column to the face;
+//   1. the three columns of F~ = F J* go through LDS (Q >= 3: all three in
+//      one round behind one barrier, while the first face task's neighbour
+//      line is already in flight; Q = 2: one column per round), every
+//      solution point accumulates its sum-factorised Q-point derivatives
+//      and every flux point extrapolates its direction's column to the face;
 //   2. per flux point of the cells' 6 faces: own trace (LDS), neighbour trace
 //      (the neighbour's line of Q points, gathered through L2), upwind flux
 //      and flux gap.  Each interior face is solved by both of its cells with
@@ -31,7 +32,7 @@ namespace mcfd {
 
 namespace {
 
-template <int Q, int Cells = 256 / (Q * Q * Q), int Threads = 256>
+template <int Q, int Cells = 256 / (Q * Q * Q), int Threads = 256, bool OneRound = false>
 struct Shape {
   static constexpr int kPoints = Q * Q * Q;        // solution points per cell
   static constexpr int kFacePoints = Q * Q;        // flux points per face
@@ -48,7 +49,10 @@ struct Shape {
   static constexpr int kSpan = (Q - 1) * (kSI + kSJ + 1) + 1;
   static constexpr int kPad = kSpan | 1;           // odd row stride per component
   static constexpr int kFacePad = kFaceTasks + 1;
-  static constexpr int kScratch = 5 * (kPad > kFacePad ? kPad : kFacePad);
+  // per cell: the F~ column(s) in flight (one direction, or all three with
+  // OneRound), later the flux gaps
+  static constexpr int kColumns = (OneRound ? 15 : 5) * kPad;
+  static constexpr int kScratch = kColumns > 5 * kFacePad ? kColumns : 5 * kFacePad;
   static constexpr int kTasks = (kCells * kFaceTasks + Threads - 1) / Threads;  // per thread
   static_assert(kCells >= 1, "at most one workgroup of solution points per cell");
   // states, F~ columns / flux gaps, mixed reference values, face area scales
@@ -151,10 +155,17 @@ __device__ __forceinline__ void LocalFluxColumn(State const &c, State const &p, 
 // the divergence); each direction's FP32 partial is accumulated in FP64.
 // States, pressure recovery, traces, the Riemann solve, the flux gaps and
 // the RK combination stay FP64.
-template <int Q, int Cells, int Threads, int kRiemann, bool kDiag, bool kMixed>
+//
+// kOneRound: all three F~ columns cross LDS in one round (three times the
+// column space; one barrier instead of six, two in the mixed form); the
+// arithmetic and its order are those of the three-round form, so both give
+// bitwise the same results.  kPrefetch (with kOneRound): the neighbour lines
+// of that many face-task slots are loaded before the passes.
+template <int Q, int Cells, int Threads, int kRiemann, bool kDiag, bool kMixed, bool kOneRound = false,
+          int kPrefetch = 0>
 __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTables &lt, int64_t block,
                                                double *lds) {
-  using S = Shape<Q, Cells, Threads>;
+  using S = Shape<Q, Cells, Threads, kOneRound>;
   constexpr int NP = S::kPoints, NFP = S::kFacePoints, C = S::kCells;
   constexpr int PAD = S::kPad, FPAD = S::kFacePad, T = S::kTasks;
   double *s_u = lds;                          // states, SoA per cell
@@ -239,30 +250,76 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
     }
   }
 
-  // ---- three passes: one column of F~ at a time through LDS
+  // neighbour line of face task t (Q nodes of the neighbour cell along the
+  // face normal, in the orientation of this cell's flux point)
+  auto gather = [&](int t, double (*raw)[5]) {
+    const int32_t nb = t_nb[t], info = t_info[t];
+    const int ia = t_a[t], ib = t_b[t];
+    const int nf = info & 7, orient = (info >> 3) & 7;
+    const int s = (orient & 1) ? ib : ia, tt = (orient & 1) ? ia : ib;
+    const int na = (orient & 2) ? Q - 1 - s : s;
+    const int nbb = (orient & 4) ? Q - 1 - tt : tt;
+    const LineIndex ln = FaceLineIndex(Q, nf);
+    const double *src = a.us + ((int64_t)nb * NP + na * ln.a_mul + nbb * ln.b_mul) * 5;
+#pragma unroll
+    for (int m = 0; m < Q; ++m)
+#pragma unroll
+      for (int k = 0; k < 5; ++k) raw[m][k] = src[m * ln.stride * 5 + k];
+  };
+  constexpr int PF = kPrefetch < T ? kPrefetch : T;
+  double pre[PF > 0 ? PF : 1][Q][5];
+
+  // ---- three passes: one column of F~ at a time through LDS (or all three
+  // written at once with kOneRound)
   double div[5] = { 0, 0, 0, 0, 0 };
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    __syncthreads();
-    double col[5];
+  if constexpr (kOneRound) {
+    if constexpr (kMixed) __syncthreads();  // the FP64 references of the deviations
     if (node_active) {
-      LocalFluxColumn<kDiag>(cu, pu, js, d, col);
-      if constexpr (kMixed) {
 #pragma unroll
-        for (int k = 0; k < 5; ++k)
-          s_x32[ncell * S::kScratch + k * PAD + spos] = (float)(col[k] - s_ref[(d * C + ncell) * 5 + k]);
-      } else {
+      for (int d = 0; d < 3; ++d) {
+        double col[5];
+        LocalFluxColumn<kDiag>(cu, pu, js, d, col);
+        if constexpr (kMixed) {
 #pragma unroll
-        for (int k = 0; k < 5; ++k) s_x[ncell * S::kScratch + k * PAD + spos] = col[k];
+          for (int k = 0; k < 5; ++k)
+            s_x32[ncell * S::kScratch + (5 * d + k) * PAD + spos] = (float)(col[k] - s_ref[(d * C + ncell) * 5 + k]);
+        } else {
+#pragma unroll
+          for (int k = 0; k < 5; ++k) s_x[ncell * S::kScratch + (5 * d + k) * PAD + spos] = col[k];
+        }
       }
     }
     __syncthreads();
+    // the first face tasks' neighbour lines, in flight during the passes
+#pragma unroll
+    for (int t = 0; t < PF; ++t)
+      if (t_on[t] && t_nb[t] >= 0) gather(t, pre[t]);
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int cb = kOneRound ? 5 * d * PAD : 0;  // column d within the cell's scratch
+    if constexpr (!kOneRound) {
+      __syncthreads();
+      double col[5];
+      if (node_active) {
+        LocalFluxColumn<kDiag>(cu, pu, js, d, col);
+        if constexpr (kMixed) {
+#pragma unroll
+          for (int k = 0; k < 5; ++k)
+            s_x32[ncell * S::kScratch + k * PAD + spos] = (float)(col[k] - s_ref[(d * C + ncell) * 5 + k]);
+        } else {
+#pragma unroll
+          for (int k = 0; k < 5; ++k) s_x[ncell * S::kScratch + k * PAD + spos] = col[k];
+        }
+      }
+      __syncthreads();
+    }
     if (node_active) {
       const int id = d == 0 ? ni : (d == 1 ? nj : nk);
       const int stride = d == 0 ? S::kSI : (d == 1 ? S::kSJ : 1);
       const int line0 = spos - id * stride;
       if constexpr (kMixed) {
-        const float *sx = s_x32 + ncell * S::kScratch;
+        const float *sx = s_x32 + ncell * S::kScratch + cb;
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
           float acc = 0.0f;
@@ -271,7 +328,7 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
           div[k] += (double)acc;
         }
       } else {
-        const double *sx = s_x + ncell * S::kScratch;
+        const double *sx = s_x + ncell * S::kScratch + cb;
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
           double acc = div[k];
@@ -288,7 +345,7 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
         const int l0 = t_a[t] * li.a_mul + t_b[t] * li.b_mul;
         const double *lw = lt.lend[li.side];
         if constexpr (kMixed) {
-          const float *sx = s_x32 + t_cell[t] * S::kScratch;
+          const float *sx = s_x32 + t_cell[t] * S::kScratch + cb;
 #pragma unroll
           for (int k = 0; k < 5; ++k) {
             float acc = (float)lw[0] * sx[k * PAD + l0];
@@ -297,7 +354,7 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
             fn[t][k] = s_ref[(d * C + t_cell[t]) * 5 + k] + (double)acc;
           }
         } else {
-          const double *sx = s_x + t_cell[t] * S::kScratch;
+          const double *sx = s_x + t_cell[t] * S::kScratch + cb;
 #pragma unroll
           for (int k = 0; k < 5; ++k) {
             double vals[Q];
@@ -343,19 +400,17 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
     State fu;
     double sgn = 1.0;
     if (nb >= 0) {
-      const int nf = info & 7, orient = (info >> 3) & 7;
       const bool holder = (info >> 6) & 1;
-      const int s = (orient & 1) ? ib : ia, tt = (orient & 1) ? ia : ib;
-      const int na = (orient & 2) ? Q - 1 - s : s;
-      const int nbb = (orient & 4) ? Q - 1 - tt : tt;
-      const LineIndex ln = FaceLineIndex(Q, nf);
-      const double *nw = lt.lend[ln.side];
-      const double *src = a.us + ((int64_t)nb * NP + na * ln.a_mul + nbb * ln.b_mul) * 5;
+      const double *nw = lt.lend[FaceLineIndex(Q, info & 7).side];
       double raw[Q][5];
+      if (kOneRound && t < PF) {
 #pragma unroll
-      for (int m = 0; m < Q; ++m)
+        for (int m = 0; m < Q; ++m)
 #pragma unroll
-        for (int k = 0; k < 5; ++k) raw[m][k] = src[m * ln.stride * 5 + k];
+          for (int k = 0; k < 5; ++k) raw[m][k] = pre[t < PF ? t : 0][m][k];
+      } else {
+        gather(t, raw);
+      }
       State other;
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
@@ -460,9 +515,9 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
   }
 }
 
-template <int Q, int Cells, int Threads, int kRiemann, bool kDiag, bool kMixed>
+template <int Q, int Cells, int Threads, int kRiemann, bool kDiag, bool kMixed, bool kOneRound, int kPrefetch>
 __global__ __launch_bounds__(Threads) void FrHexStageKernel(FrHexArgs a, LineTables lt) {
-  __shared__ double lds[Shape<Q, Cells, Threads>::kLdsDoubles];
+  __shared__ double lds[Shape<Q, Cells, Threads, kOneRound>::kLdsDoubles];
   // The line tables are indexed by lane-varying node indices: read from the
   // kernel arguments they would be vector loads on the critical path of every
   // pass, so they are staged in LDS once (the body's first barrier orders it).
@@ -472,7 +527,7 @@ __global__ __launch_bounds__(Threads) void FrHexStageKernel(FrHexArgs a, LineTab
     reinterpret_cast<double *>(&s_lt)[threadIdx.x] = reinterpret_cast<const double *>(&lt)[threadIdx.x];
   int64_t block = LogicalBlock(a.xcd_remap);
   if (a.tile_order) block = a.tile_order[block];
-  FrHexStageBody<Q, Cells, Threads, kRiemann, kDiag, kMixed>(a, s_lt, block, lds);
+  FrHexStageBody<Q, Cells, Threads, kRiemann, kDiag, kMixed, kOneRound, kPrefetch>(a, s_lt, block, lds);
 }
 
 // Single-mesh launch shape per Q: 256 threads, one per solution point.  At
@@ -483,6 +538,10 @@ __global__ __launch_bounds__(Threads) void FrHexStageKernel(FrHexArgs a, LineTab
 template <int Q> struct StageTile {
   static constexpr int kCells = Shape<Q>::kCells;
   static constexpr int kThreads = 256;
+  static constexpr bool kOneRound = Q >= 3;  // at Q = 2 the column space would cost a workgroup per CU
+  // face tasks whose neighbour lines are issued before the passes: one, as
+  // the registers of the three-waves-per-SIMD budget allow
+  static constexpr int kPrefetch = 1;
 };
 
 // Tile variants per Q (cells per workgroup) that the batched launch and its
@@ -576,20 +635,22 @@ __global__ __launch_bounds__(256) void FrHexTimeStepKernel(const double *u, cons
 template <int Q, int R>
 hipError_t Launch(const FrHexArgs &args, const LineTables &lt, hipStream_t stream) {
   constexpr int C = StageTile<Q>::kCells, W = StageTile<Q>::kThreads;
+  constexpr bool O = StageTile<Q>::kOneRound;
+  constexpr int PF = StageTile<Q>::kPrefetch;
   int64_t blocks = (args.n_cells + C - 1) / C;
   if (blocks > 0x7fffffff) return hipErrorInvalidValue;
   const dim3 grid((unsigned)blocks), block(W);
   if (args.mixed) {
     if (args.diag_metric) {
-      hipLaunchKernelGGL((FrHexStageKernel<Q, C, W, R, true, true>), grid, block, 0, stream, args, lt);
+      hipLaunchKernelGGL((FrHexStageKernel<Q, C, W, R, true, true, O, PF>), grid, block, 0, stream, args, lt);
     } else {
-      hipLaunchKernelGGL((FrHexStageKernel<Q, C, W, R, false, true>), grid, block, 0, stream, args, lt);
+      hipLaunchKernelGGL((FrHexStageKernel<Q, C, W, R, false, true, O, PF>), grid, block, 0, stream, args, lt);
     }
   } else {
     if (args.diag_metric) {
-      hipLaunchKernelGGL((FrHexStageKernel<Q, C, W, R, true, false>), grid, block, 0, stream, args, lt);
+      hipLaunchKernelGGL((FrHexStageKernel<Q, C, W, R, true, false, O, PF>), grid, block, 0, stream, args, lt);
     } else {
-      hipLaunchKernelGGL((FrHexStageKernel<Q, C, W, R, false, false>), grid, block, 0, stream, args, lt);
+      hipLaunchKernelGGL((FrHexStageKernel<Q, C, W, R, false, false, O, PF>), grid, block, 0, stream, args, lt);
     }
   }
   return hipGetLastError();
