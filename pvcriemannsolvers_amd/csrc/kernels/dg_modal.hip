@@ -13,12 +13,13 @@
 // quadrature points of its faces), one thread per task, volume tasks of all
 // G cells first.
 //   phase 0  the cells' Taylor coefficients and geometry are staged in LDS;
+//            the Gram-Schmidt factors and the state / U0 of phase 3 are
+//            loaded at the same time, so one round trip serves the pass;
 //   phase 1  volume task: state u = Ct T(y) at the point, flux matrix times
 //            the global weight -> LDS;  face task: -/+ w Fhat read from fhat
 //            and the point's offset Y -> LDS;
 //   phase 2  one thread per (cell, component, Taylor monomial) sums the
-//            moments Rt over the cell's tasks, while the Gram-Schmidt factors
-//            and the state / U0 of phase 3 arrive;
+//            moments Rt over the cell's tasks;
 //   phase 3  one thread per (cell, basis function, component): R = Rt S^T,
 //            the SSP-RK3 combination (temporal/rk.hpp:31-49 operation order)
 //            and the store;
@@ -153,6 +154,23 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
     }
   }
 
+  // the Gram-Schmidt factors S and the state and U0 of the phase-3 items,
+  // issued with the staging loads so that one round trip serves all of them
+  double s_pf[kSIter];
+#pragma unroll
+  for (int it = 0; it < kSIter; ++it) {
+    const int w = tid + 256 * it;
+    s_pf[it] = w < nc * PK ? a.basis[c0 * PK + w] : 0.0;
+  }
+  double us_pf[kItems], u0_pf[kItems];
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const int w = tid + 256 * it;
+    const bool on = w < nc * 5 * N;
+    us_pf[it] = on && a.stage >= 1 ? a.us[c0 * 5 * N + w] : 0.0;
+    u0_pf[it] = on && a.stage >= 2 ? a.u0[c0 * 5 * N + w] : 0.0;
+  }
+
   // ---- phase 0: stage the cells' data in LDS
   for (int w = tid; w < nc * 5 * N; w += 256) s_ct[w] = a.ct[c0 * 5 * N + w];
   for (int w = tid; w < nc * GS; w += 256) s_geo[w] = a.geo[c0 * GS + w];
@@ -193,23 +211,8 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
   }
   __syncthreads();
 
-  // ---- phase 2: the Gram-Schmidt factors S and the state and U0 of the
-  // phase-3 items are fetched first, so that their latency overlaps the
-  // moments; S goes to the freed LDS of the own data at the end
-  double s_pf[kSIter];
-#pragma unroll
-  for (int it = 0; it < kSIter; ++it) {
-    const int w = tid + 256 * it;
-    s_pf[it] = w < nc * PK ? a.basis[c0 * PK + w] : 0.0;
-  }
-  double us_pf[kItems], u0_pf[kItems];
-#pragma unroll
-  for (int it = 0; it < kItems; ++it) {
-    const int w = tid + 256 * it;
-    const bool on = w < nc * 5 * N;
-    us_pf[it] = on && a.stage >= 1 ? a.us[c0 * 5 * N + w] : 0.0;
-    u0_pf[it] = on && a.stage >= 2 ? a.u0[c0 * 5 * N + w] : 0.0;
-  }
+  // ---- phase 2: Taylor moments; S goes to the freed LDS of the own data
+  // at the end
   // Taylor moments Rt[k][j], items ordered cell-major so that the lanes of a
   // wavefront read the same task records (LDS broadcast)
   for (int w = tid; w < nc * 5 * N; w += 256) {
@@ -456,60 +459,74 @@ __device__ __forceinline__ void EvalTaylorRhoE(const double *ct, const double *y
 
 __global__ void DgLimiterResetKernel(int32_t *n_troubled) { *n_troubled = 0; }
 
-// limiter/weno.hpp:230-258 (weno::Eigen::IsNotSmooth) on rho and E; every
-// flagged cell is also appended to the troubled-cell list (one atomic per
-// wavefront; the list order is irrelevant, each cell's reconstruction only
-// depends on the cell).
-template <bool kHex, int P>
-__device__ __forceinline__ bool IsNotSmooth(const DgLimiterArgs &a, int64_t c) {
-  constexpr int NA = kHex ? 6 : 4;  // neighbours at most
-  constexpr int N = P == 1 ? 4 : 10, PACKED = N * (N + 1) / 2;
-  const double *gc = a.geo + c * kDgGeoStride;
-  const double s00 = a.basis[c * PACKED];
-  const double *cf = a.u + c * 5 * N;
-  const double *ct = a.ct + c * 5 * N;
-  double max_abs0 = fmax(1e-9, fabs(cf[0] * s00)), max_abs4 = fmax(1e-9, fabs(cf[4] * s00));
-  double sum0 = 0.0, sum4 = 0.0;
-  const double mine0 = ct[0], mine4 = ct[4];  // u(centre) = Ct T(0)
-  const int na = a.n_adj[c];
-  // A fixed trip count lets every neighbour's loads issue at once; the
-  // unused slots read the cell itself and are not accumulated.
-  int64_t nbs[NA];
-#pragma unroll
-  for (int i = 0; i < NA; ++i) nbs[i] = i < na ? (int64_t)a.adj[c * 6 + i] : c;
-#pragma unroll
-  for (int i = 0; i < NA; ++i) {
-    const int64_t o = nbs[i];
-    const double *go = a.geo + o * kDgGeoStride;
-    const double y[3] = { gc[11] - go[11], gc[12] - go[12], gc[13] - go[13] };
-    double rho, e;
-    EvalTaylorRhoE<P>(a.ct + o * 5 * N, y, &rho, &e);
-    const double so = a.basis[o * PACKED];
-    const double u0 = a.u[o * 5 * N], u4 = a.u[o * 5 * N + 4];
-    if (i < na) {
-      sum0 += fabs(mine0 - rho);
-      sum4 += fabs(mine4 - e);
-      max_abs0 = fmax(max_abs0, fabs(u0 * so));
-      max_abs4 = fmax(max_abs4, fabs(u4 * so));
-    }
-  }
-  double divisor = pow(gc[14], (P + 1.0) / 2.0 / 3);
-  divisor *= na;
-  const double reference = P < 3 ? 1.0 : 3.0;
-  return sum0 / max_abs0 / divisor > reference || sum4 / max_abs4 / divisor > reference;
-}
-
+// limiter/weno.hpp:230-258 (weno::Eigen::IsNotSmooth) on rho and E, with L
+// lanes per cell: lane i evaluates neighbour i's polynomial at the cell's
+// centre, and the group's first lane sums the jumps in neighbour order (the
+// reference's order, so the flags are bitwise the same) and takes the
+// maxima.  Every flagged cell is also appended to the troubled-cell list (one
+// atomic per wavefront; the list order is irrelevant, each cell's
+// reconstruction only depends on the cell).
 template <bool kHex, int P>
 __global__ __launch_bounds__(256) void DgWenoFlagsKernel(DgLimiterArgs a) {
-  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  constexpr int NA = kHex ? 6 : 4;  // neighbours at most
+  constexpr int L = kHex ? 8 : 4;   // lanes per cell, a power of two dividing 64
+  constexpr int N = P == 1 ? 4 : 10, PACKED = N * (N + 1) / 2;
+  const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t c = w / L;
+  const int i = (int)(w & (L - 1));
+  const bool on = c < a.n_cells;
+  const bool all = a.mode == 2 || a.mode == 4;
+  int na = 0;
+  double jump0 = 0.0, jump4 = 0.0, abs0 = 0.0, abs4 = 0.0;  // neighbour i's terms
+  if (on && !all) {
+    na = a.n_adj[c];
+    if (i < na) {
+      const int64_t o = a.adj[c * 6 + i];
+      const double *gc = a.geo + c * kDgGeoStride, *go = a.geo + o * kDgGeoStride;
+      const double y[3] = { gc[11] - go[11], gc[12] - go[12], gc[13] - go[13] };
+      double rho, e;
+      EvalTaylorRhoE<P>(a.ct + o * 5 * N, y, &rho, &e);
+      const double so = a.basis[o * PACKED];
+      jump0 = fabs(a.ct[c * 5 * N] - rho);  // u(centre) = Ct T(0)
+      jump4 = fabs(a.ct[c * 5 * N + 4] - e);
+      abs0 = fabs(a.u[o * 5 * N] * so);
+      abs4 = fabs(a.u[o * 5 * N + 4] * so);
+    }
+  }
+  const int lane = (int)(threadIdx.x & 63), first = lane & ~(L - 1);
+  double sum0 = 0.0, sum4 = 0.0, max0 = 0.0, max4 = 0.0;
+#pragma unroll
+  for (int j = 0; j < NA; ++j) {
+    const double v0 = __shfl(jump0, first + j), v4 = __shfl(jump4, first + j);
+    const double m0 = __shfl(abs0, first + j), m4 = __shfl(abs4, first + j);
+    if (j < na) {
+      sum0 += v0;
+      sum4 += v4;
+      max0 = j == 0 ? m0 : fmax(max0, m0);
+      max4 = j == 0 ? m4 : fmax(max4, m4);
+    }
+  }
   bool rough = false;
-  if (c < a.n_cells) {
-    rough = (a.mode == 2 || a.mode == 4) ? true : IsNotSmooth<kHex, P>(a, c);
+  if (on && i == 0) {
+    if (all) {
+      rough = true;
+    } else {
+      const double *gc = a.geo + c * kDgGeoStride;
+      const double s00 = a.basis[c * PACKED];
+      double max_abs0 = fmax(1e-9, fabs(a.u[c * 5 * N] * s00)), max_abs4 = fmax(1e-9, fabs(a.u[c * 5 * N + 4] * s00));
+      if (na > 0) {
+        max_abs0 = fmax(max_abs0, max0);
+        max_abs4 = fmax(max_abs4, max4);
+      }
+      double divisor = pow(gc[14], (P + 1.0) / 2.0 / 3);
+      divisor *= na;
+      const double reference = P < 3 ? 1.0 : 3.0;
+      rough = sum0 / max_abs0 / divisor > reference || sum4 / max_abs4 / divisor > reference;
+    }
     a.flags[c] = rough ? 1 : 0;
   }
   const unsigned long long ballot = __ballot(rough);
   if (ballot == 0) return;
-  const int lane = (int)(threadIdx.x & 63);
   const int leader = __ffsll((long long)ballot) - 1;
   int base = 0;
   if (lane == leader) base = atomicAdd(a.n_troubled, __popcll(ballot));
@@ -905,7 +922,8 @@ hipError_t LaunchDgLimiter(int cell_type, int degree, const DgLimiterArgs &a, hi
   if (a.n_cells < 1 || a.mode == 0) return hipSuccess;
   const bool hex = cell_type == 0;
   const int n = degree == 1 ? 4 : 10;
-  const int64_t b_flags = (a.n_cells + 255) / 256;
+  const int64_t b_flags = (a.n_cells * (hex ? 8 : 4) + 255) / 256;  // lanes per cell as DgWenoFlagsKernel
+  if (b_flags > 0x7fffffff) return hipErrorInvalidValue;
   // one 64-thread workgroup per troubled cell at a time, striding over the list
   const int64_t b_proj = std::min<int64_t>(a.n_cells, 8192);
   const int64_t b_comb = (a.n_cells * 5 + 255) / 256;
