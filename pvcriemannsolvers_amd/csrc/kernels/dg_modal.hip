@@ -36,16 +36,11 @@ namespace mcfd {
 namespace {
 
 // LDS record of one volume task: the weighted flux matrix as F[d][k] (15),
-// a zero, then Y = (y, 1) (4).  A face task: the weighted flux (5), then Y
-// (4).  With the constant factor and the zero in place, every Taylor moment
-// of degree <= 2 is two lane-indexed reads per point:
-//   grad(y_a y_b) . F[k] = F[a][k] Y[b] + F[b][k] Y[a],   T = Y[a] Y[b],
-// where a factor index 3 stands for the constant 1 (basis/taylor.hpp:204-208)
-// and F[3][k] reads the zero.
+// a pad, then the point's offset y (3) from kSlotY.  A face task: the
+// weighted flux (5), then y (3).
 constexpr int kSlot = 20;
-constexpr int kSlotZero = 15, kSlotY = 16;
+constexpr int kSlotY = 16;
 constexpr int kFaceSlot = 9;
-__device__ __forceinline__ int FluxIndex(int d, int k) { return d == 3 ? kSlotZero : 5 * d + k; }
 
 template <bool kHex, int P>
 struct DgShape {
@@ -86,23 +81,45 @@ __device__ __forceinline__ State EvalTaylor(const double *ct, const double *y) {
   return u;
 }
 
-// Taylor monomial j of degree <= 2 (basis/taylor.hpp:204-208: 1, x, y, z,
-// xx, xy, xz, yy, yz, zz) as the product Y[a] Y[b] of two factors of
-// Y = (x, y, z, 1).
-struct MonomialCode {
-  int a, b;
-};
-__device__ __forceinline__ MonomialCode MonomialOf(int j) {
-  MonomialCode m;
-  m.a = j == 0 ? 3 : (j <= 3 ? j - 1 : (j <= 6 ? 0 : (j <= 8 ? 1 : 2)));
-  m.b = j <= 3 ? 3 : (j == 4 ? 0 : (j == 5 || j == 7 ? 1 : 2));
-  return m;
+// Lanes per (cell, component) in the moment phase of a 256-thread
+// workgroup of G cells: the largest power of two up to 16 (one DPP row)
+// with G * 5 * NPART <= 256.
+template <int G>
+constexpr int DgMomentParts() {
+  int p = 16;
+  while (p > 1 && G * 5 * p > 256) p /= 2;
+  return p;
+}
+
+// v from the lane given by a DPP control (within a row of 16 lanes).
+template <int kCtrl>
+__device__ __forceinline__ double DppMove(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), kCtrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), kCtrl, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over each aligned group of NPART lanes (NPART = 1, 2, 4, 8 or 16), by
+// a butterfly: lane ^ 1 (quad_perm 1,0,3,2), lane ^ 2 (quad_perm 2,3,0,1),
+// the other quad of the half row (row_half_mirror), the other half of the
+// row (row_ror 8).  Every lane of a group ends with the same bits, since each
+// step adds two values whose order does not matter.  All lanes of the
+// wavefront must execute it.
+template <int NPART>
+__device__ __forceinline__ double GroupSum(double v) {
+  if constexpr (NPART >= 2) v += DppMove<0xB1>(v);
+  if constexpr (NPART >= 4) v += DppMove<0x4E>(v);
+  if constexpr (NPART >= 8) v += DppMove<0x141>(v);
+  if constexpr (NPART >= 16) v += DppMove<0x128>(v);
+  return v;
 }
 
 // The face fluxes w Fhat were computed once per face point by
-// DgFaceFluxKernel into a.fhat; the face tasks read them.
+// DgFaceFluxKernel into a.fhat; the face tasks read them.  Tetrahedra run at
+// 5 waves per SIMD (the LDS of 5 workgroups fits in a CU): the few bytes the
+// register bound spills cost less than the fifth wave gains.
 template <bool kHex, int P>
-__global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
+__global__ __launch_bounds__(256, kHex ? 4 : 5) void DgModalStageKernel(DgModalArgs a) {
   using S = DgShape<kHex, P>;
   constexpr int N = S::kN, NQ = S::kNQ, NF = S::kNF, NFP = S::kNFP, T = S::kT, G = S::kG, PK = S::kPacked;
   constexpr int GS = kDgGeoStride;
@@ -192,10 +209,8 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
       for (int k = 0; k < 5; ++k)
 #pragma unroll
         for (int d = 0; d < 3; ++d) slot[5 * d + k] = f[k][d] * w;
-      slot[kSlotZero] = 0.0;
 #pragma unroll
       for (int d = 0; d < 3; ++d) slot[kSlotY + d] = y[d];
-      slot[kSlotY + 3] = 1.0;
     } else {
       // dg/general.hpp:80-88: -w Fhat into the holder, +w Fhat into the sharer
       double *slot = s_face + (g * NF * NFP + (r - NQ)) * kFaceSlot;
@@ -206,33 +221,67 @@ __global__ __launch_bounds__(256, 4) void DgModalStageKernel(DgModalArgs a) {
       for (int k = 0; k < 5; ++k) slot[k] = holder ? -fw[k] : fw[k];
 #pragma unroll
       for (int d = 0; d < 3; ++d) slot[5 + d] = y[d];
-      slot[8] = 1.0;
     }
   }
   __syncthreads();
 
   // ---- phase 2: Taylor moments; S goes to the freed LDS of the own data
   // at the end
-  // Taylor moments Rt[k][j], items ordered cell-major so that the lanes of a
-  // wavefront read the same task records (LDS broadcast)
-  for (int w = tid; w < nc * 5 * N; w += 256) {
-    const int gc = w / (5 * N), rem = w - gc * 5 * N, j = rem / 5, k = rem - (rem / 5) * 5;
-    const MonomialCode m = MonomialOf(j);
-    const double *vb = s_vol + gc * NQ * kSlot;
-    const double *vf_a = vb + FluxIndex(m.a, k), *vf_b = vb + FluxIndex(m.b, k);
-    const double *vy_a = vb + kSlotY + m.a, *vy_b = vb + kSlotY + m.b;
-    const double *ff = s_face + gc * NF * NFP * kFaceSlot + k;
-    const double *fy_a = s_face + gc * NF * NFP * kFaceSlot + 5 + m.a, *fy_b = fy_a + (m.b - m.a);
-    double acc = 0.0;
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      acc = __fma_rn(vf_a[q * kSlot], vy_b[q * kSlot], acc);
-      acc = __fma_rn(vf_b[q * kSlot], vy_a[q * kSlot], acc);
+  // Taylor moments Rt[k][j].  Lane (cell, component, part) accumulates all N
+  // moments of its component over the points part, part + NPART, ... of the
+  // cell (volume points first, then face points), with the per-point FMA
+  // chains of the monomial table (basis/taylor.hpp:204-208):
+  //   volume: grad(Y_a Y_b) . F = F_a Y_b + F_b Y_a,  face: Fhat Y_a Y_b;
+  // the NPART adjacent lanes then sum their partial moments by a fixed
+  // butterfly, so every lane of the group holds the same bits.
+  {
+    constexpr int NPART = DgMomentParts<G>();
+    const int grp = tid / NPART, part = tid - (tid / NPART) * NPART;
+    const int gc = grp / 5, k = grp - (grp / 5) * 5;
+    double mo[10] = { 0, 0, 0, 0, 0, 0, 0, 0, 0, 0 };
+    if (gc < nc) {
+      const double *vb = s_vol + gc * NQ * kSlot;
+      for (int q = part; q < NQ; q += NPART) {
+        const double *r = vb + q * kSlot;
+        const double f0 = r[k], f1 = r[5 + k], f2 = r[10 + k];
+        const double y0 = r[kSlotY], y1 = r[kSlotY + 1], y2 = r[kSlotY + 2];
+        mo[1] += f0;
+        mo[2] += f1;
+        mo[3] += f2;
+        if constexpr (N == 10) {
+          mo[4] = __fma_rn(f0, y0, __fma_rn(f0, y0, mo[4]));
+          mo[5] = __fma_rn(f1, y0, __fma_rn(f0, y1, mo[5]));
+          mo[6] = __fma_rn(f2, y0, __fma_rn(f0, y2, mo[6]));
+          mo[7] = __fma_rn(f1, y1, __fma_rn(f1, y1, mo[7]));
+          mo[8] = __fma_rn(f2, y1, __fma_rn(f1, y2, mo[8]));
+          mo[9] = __fma_rn(f2, y2, __fma_rn(f2, y2, mo[9]));
+        }
+      }
+      const double *fb = s_face + gc * NF * NFP * kFaceSlot;
+      for (int q = (part + NPART - NQ % NPART) % NPART; q < NF * NFP; q += NPART) {
+        const double *r = fb + q * kFaceSlot;
+        const double ff = r[k], y0 = r[5], y1 = r[6], y2 = r[7];
+        mo[0] += ff;
+        mo[1] = __fma_rn(ff, y0, mo[1]);
+        mo[2] = __fma_rn(ff, y1, mo[2]);
+        mo[3] = __fma_rn(ff, y2, mo[3]);
+        if constexpr (N == 10) {
+          mo[4] = __fma_rn(ff, y0 * y0, mo[4]);
+          mo[5] = __fma_rn(ff, y0 * y1, mo[5]);
+          mo[6] = __fma_rn(ff, y0 * y2, mo[6]);
+          mo[7] = __fma_rn(ff, y1 * y1, mo[7]);
+          mo[8] = __fma_rn(ff, y1 * y2, mo[8]);
+          mo[9] = __fma_rn(ff, y2 * y2, mo[9]);
+        }
+      }
     }
 #pragma unroll
-    for (int q = 0; q < NF * NFP; ++q)
-      acc = __fma_rn(ff[q * kFaceSlot], fy_a[q * kFaceSlot] * fy_b[q * kFaceSlot], acc);
-    s_rt[w] = acc;
+    for (int j = 0; j < N; ++j) mo[j] = GroupSum<NPART>(mo[j]);
+    if (gc < nc) {
+#pragma unroll
+      for (int j = 0; j < N; ++j)
+        if (j % NPART == part) s_rt[gc * 5 * N + 5 * j + k] = mo[j];
+    }
   }
 #pragma unroll
   for (int it = 0; it < kSIter; ++it) {
