@@ -530,14 +530,18 @@ __global__ __launch_bounds__(Threads) void FrHexStageKernel(FrHexArgs a, LineTab
   FrHexStageBody<Q, Cells, Threads, kRiemann, kDiag, kMixed, kOneRound, kPrefetch>(a, s_lt, block, lds);
 }
 
-// Single-mesh launch shape per Q: 256 threads, one per solution point.  At
-// Q = 4 the face tasks (96 per cell) then fill two slots in half of the
-// wavefronts; 96 threads per cell (one task each) measured slower on MI355X,
-// since the extra wavefronts idle through the volume passes and occupy
-// wave slots that other workgroups would fill.
+// Single-mesh launch shape per Q: one thread per solution point.  Q = 4 and
+// 5 run on 128-thread workgroups (2 cells and 1 cell): the same 3 waves per
+// SIMD as 256-thread ones, with each barrier waiting on two wavefronts
+// instead of four (Q = 4: 0.93 against 0.96 ms per stage at 64^3; Q = 3 as
+// 4 cells on 128 threads leaves 20 threads idle and measured no better).  At Q = 4
+// the face tasks (96 per cell) fill two slots in half of the wavefronts; 96
+// threads per cell (one task each) measured slower on MI355X, since the extra
+// wavefronts idle through the volume passes and occupy wave slots that other
+// workgroups would fill.
 template <int Q> struct StageTile {
-  static constexpr int kCells = Shape<Q>::kCells;
-  static constexpr int kThreads = 256;
+  static constexpr int kCells = Q == 4 ? 2 : (Q == 5 ? 1 : Shape<Q>::kCells);
+  static constexpr int kThreads = (Q == 4 || Q == 5) ? 128 : 256;
   static constexpr bool kOneRound = Q >= 3;  // at Q = 2 the column space would cost a workgroup per CU
   // face tasks whose neighbour lines are issued before the passes: one, as
   // the registers of the three-waves-per-SIMD budget allow
