@@ -515,10 +515,12 @@ __global__ void DgLimiterResetKernel(int32_t *n_troubled) { *n_troubled = 0; }
 // maxima.  Every flagged cell is also appended to the troubled-cell list (one
 // atomic per wavefront; the list order is irrelevant, each cell's
 // reconstruction only depends on the cell).
+constexpr int FlagLanes(bool hex) { return hex ? 8 : 4; }  // lanes per cell, a power of two dividing 64
+
 template <bool kHex, int P>
 __global__ __launch_bounds__(256) void DgWenoFlagsKernel(DgLimiterArgs a) {
   constexpr int NA = kHex ? 6 : 4;  // neighbours at most
-  constexpr int L = kHex ? 8 : 4;   // lanes per cell, a power of two dividing 64
+  constexpr int L = FlagLanes(kHex);
   constexpr int N = P == 1 ? 4 : 10, PACKED = N * (N + 1) / 2;
   const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t c = w / L;
@@ -971,7 +973,7 @@ hipError_t LaunchDgLimiter(int cell_type, int degree, const DgLimiterArgs &a, hi
   if (a.n_cells < 1 || a.mode == 0) return hipSuccess;
   const bool hex = cell_type == 0;
   const int n = degree == 1 ? 4 : 10;
-  const int64_t b_flags = (a.n_cells * (hex ? 8 : 4) + 255) / 256;  // lanes per cell as DgWenoFlagsKernel
+  const int64_t b_flags = (a.n_cells * FlagLanes(hex) + 255) / 256;
   if (b_flags > 0x7fffffff) return hipErrorInvalidValue;
   // one 64-thread workgroup per troubled cell at a time, striding over the list
   const int64_t b_proj = std::min<int64_t>(a.n_cells, 8192);

@@ -281,8 +281,8 @@ def test_zeroing_rule_on_nonphysical_nodes(riemann):
 @pytest.mark.parametrize("degree", [3, 5])
 @pytest.mark.parametrize("riemann", ["roe", "hllc"])
 def test_partial_tile_and_largest_degree(degree, riemann):
-    """45 cells leave the last workgroup of 4 cells (Q = 4) partly empty; Q = 6
-    runs one cell of 216 solution points per workgroup."""
+    """45 cells leave the last 2-cell workgroup (Q = 4) half empty; Q = 6 runs
+    one cell of 216 solution points per workgroup."""
     mesh = meshgen.box_hex(5, 3, 3, periodic=(True, True, True), shuffle_seed=9)
     o, g = make_pair(mesh, degree, riemann)
     u = density_wave(o.node_coords(), seed=4)
