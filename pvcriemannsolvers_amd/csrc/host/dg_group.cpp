@@ -404,6 +404,12 @@ void DgGroup::Upload(hipStream_t stream) {
     nadj.Upload(h_nadj, stream);
     flags.Allocate(n_cells);
     Check(hipMemsetAsync(flags.ptr, 0, n_cells, stream), "memset");
+    std::vector<double> h_rec(size_t(n_cells) * kDgFlagRecord, 0.0);  // centre and volume; the rest per state
+    for (int64_t c = 0; c < n_cells; ++c) {
+      for (int d = 0; d < 3; ++d) h_rec[c * kDgFlagRecord + kDgFlagCentre + d] = h_geo[c * kDgGeoStride + 11 + d];
+      h_rec[c * kDgFlagRecord + kDgFlagVolume] = h_geo[c * kDgGeoStride + 14];
+    }
+    flag_rec.Upload(h_rec, stream);
     troubled.Allocate(size_t(n_cells) + 1);
     proj.Allocate(size_t(n_cells) * 7 * 5 * n);
     beta.Allocate(size_t(n_cells) * 7 * 5);
@@ -416,14 +422,16 @@ void DgGroup::Limit(double *u, double *ct, hipStream_t stream) {
   DgLimiterArgs a;
   a.u = u; a.ct = ct; a.basis = basis.ptr; a.geo = geo.ptr;
   a.adj = adj.ptr; a.adj_face = adj_face.ptr; a.n_adj = nadj.ptr; a.faces = face_rec.ptr; a.tables = tables_d.ptr;
-  a.flags = flags.ptr; a.troubled = troubled.ptr; a.n_troubled = troubled.ptr + n_cells;
+  a.flag_rec = flag_rec.ptr; a.flags = flags.ptr; a.troubled = troubled.ptr; a.n_troubled = troubled.ptr + n_cells;
   a.proj = proj.ptr; a.beta = beta.ptr; a.eig = eig.ptr;
   a.n_cells = n_cells; a.mode = limiter; a.w0 = weno_w0; a.eps = weno_eps;
   Check(LaunchDgLimiter(cell_type, degree, a, stream), "DG limiter launch");
 }
 
 void DgGroup::SolutionChanged(hipStream_t stream) {
-  Check(LaunchDgTaylorCoefficients(degree, u0.ptr, basis.ptr, n_cells, ct0.ptr, stream), "DG Taylor launch");
+  Check(LaunchDgTaylorCoefficients(degree, u0.ptr, basis.ptr, n_cells, ct0.ptr, limiter ? flag_rec.ptr : nullptr,
+                                   stream),
+        "DG Taylor launch");
   Limit(u0.ptr, ct0.ptr, stream);  // spatial/with_limiter.hpp:56-59
 }
 
@@ -442,6 +450,7 @@ void DgGroup::Stage(int stage, double dt, hipStream_t stream, unsigned long long
   a.n_cells = n_cells; a.dt = dt; a.stage = stage;
   a.dt_min_bits = stage > 0 && !limiter ? dt_min_bits : nullptr;
   a.fhat = fhat.ptr;
+  a.flag_rec = limiter && stage > 0 ? flag_rec.ptr : nullptr;
   a.face_cells = face_cells.ptr;
   a.n_faces = (int64_t)faces.size();
   Check(LaunchDgModalStage(cell_type, degree, riemann, a, stream), "DG stage launch");

@@ -45,6 +45,7 @@ struct DgGroup {
   std::vector<int32_t> h_adj, h_adj_face, h_nadj;  // Cell::adj_cells_/adj_faces_ in face order, [cell][6]
   DeviceArray<int32_t> adj, adj_face, nadj;
   DeviceArray<uint8_t> flags;
+  DeviceArray<double> flag_rec;        // [cell][kDgFlagRecord], the detector's view of the current state
   DeviceArray<int32_t> troubled;       // flagged cells of the last limiter pass, then their count
   DeviceArray<double> proj, beta, eig;
 

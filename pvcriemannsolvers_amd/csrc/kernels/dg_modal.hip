@@ -18,12 +18,13 @@
 //   phase 1  volume task: state u = Ct T(y) at the point, flux matrix times
 //            the global weight -> LDS;  face task: -/+ w Fhat read from fhat
 //            and the point's offset Y -> LDS;
-//   phase 2  one thread per (cell, component, Taylor monomial) sums the
-//            moments Rt over the cell's tasks;
+//   phase 2  the Taylor moments Rt: a group of lanes per (cell, component),
+//            each lane over a subset of the cell's tasks, summed by DPP;
 //   phase 3  one thread per (cell, basis function, component): R = Rt S^T,
 //            the SSP-RK3 combination (temporal/rk.hpp:31-49 operation order)
 //            and the store;
-//   phase 4  the new state's Taylor coefficients for the next pass;
+//   phase 4  the new state's Taylor coefficients for the next pass (and,
+//            with a limiter, the WENO detector's record of the new state);
 //   phase 5  (optional) the time-step reduction of the new state.
 #include <hip/hip_runtime.h>
 
@@ -81,6 +82,15 @@ __device__ __forceinline__ State EvalTaylor(const double *ct, const double *y) {
   return u;
 }
 
+// The detector record's share of Taylor item (j, k) of a cell: the rho (k =
+// 0) and E (k = 4) coefficients, and with j = 0 the averages u[0][k] *
+// S[0][0] (polynomial().average() of limiter/weno.hpp:232-245).
+__device__ __forceinline__ void FlagRecordItem(double *rec, int j, int k, double taylor, double u0k, double s00) {
+  if (k == 0) rec[j] = taylor;
+  if (k == 4) rec[10 + j] = taylor;
+  if (j == 0 && (k == 0 || k == 4)) rec[kDgFlagAverage + (k == 4)] = u0k * s00;
+}
+
 // Lanes per (cell, component) in the moment phase of a 256-thread
 // workgroup of G cells: the largest power of two up to 16 (one DPP row)
 // with G * 5 * NPART <= 256.
@@ -125,8 +135,8 @@ __global__ __launch_bounds__(256, kHex ? 4 : 5) void DgModalStageKernel(DgModalA
   constexpr int GS = kDgGeoStride;
   __shared__ double s_own[G * 5 * N + G * GS];  // own Taylor coefficients and geometry; then S
   __shared__ double s_rt[G * 5 * N];       // Taylor moments Rt
-  __shared__ double s_vol[G * NQ * kSlot]; // volume tasks: flux x weight, zero column, Y
-  __shared__ double s_face[G * NF * NFP * kFaceSlot];  // face tasks: -/+ w Fhat (5), Y; then new coefficients
+  __shared__ double s_vol[G * NQ * kSlot]; // volume tasks: flux x weight, pad, y
+  __shared__ double s_face[G * NF * NFP * kFaceSlot];  // face tasks: -/+ w Fhat (5), y; then the new state
   static_assert(G * 5 * N <= G * NF * NFP * kFaceSlot, "LDS aliasing");
   static_assert(G * PK <= G * 5 * N + G * GS, "LDS aliasing");
   static_assert(G * N + G <= G * NQ * kSlot, "LDS aliasing");
@@ -329,6 +339,7 @@ __global__ __launch_bounds__(256, kHex ? 4 : 5) void DgModalStageKernel(DgModalA
     for (int i = j; i < N; ++i) acc = __fma_rn(sn[5 * i + k], sc[i * (i + 1) / 2 + j], acc);
     a.ct_out[c0 * 5 * N + w] = acc;
     s_rt[w] = acc;  // for the fused time step
+    if (a.flag_rec) FlagRecordItem(a.flag_rec + (c0 + gc) * kDgFlagRecord, j, k, acc, sn[k], sc[0]);
   }
   if (!a.dt_min_bits) return;
 
@@ -445,7 +456,7 @@ __global__ __launch_bounds__(256) void DgApproximateKernel(const double *values,
 
 template <int P>
 __global__ __launch_bounds__(256) void DgTaylorKernel(const double *us, const double *basis, int64_t n_cells,
-                                                      double *ct) {
+                                                      double *ct, double *flag_rec) {
   constexpr int N = P == 1 ? 4 : 10, PACKED = N * (N + 1) / 2;
   const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (w >= n_cells * 5 * N) return;
@@ -456,6 +467,7 @@ __global__ __launch_bounds__(256) void DgTaylorKernel(const double *us, const do
   double acc = 0.0;
   for (int i = j; i < N; ++i) acc = __fma_rn(cf[5 * i + k], s[i * (i + 1) / 2 + j], acc);
   ct[w] = acc;
+  if (flag_rec) FlagRecordItem(flag_rec + c * kDgFlagRecord, j, k, acc, cf[k], s[0]);
 }
 
 template <bool kHex, int P>
@@ -486,9 +498,11 @@ __global__ __launch_bounds__(256) void DgTimeStepKernel(const double *ct, const 
 // ---------------------------------------------------------------- WENO limiter
 constexpr int kLimiterSlots = 7;  // up to 6 neighbours, then the cell itself
 
-// Components 0 (rho) and 4 (E) of Ct T(y): the same FMA chains as EvalTaylor.
-template <int P>
-__device__ __forceinline__ void EvalTaylorRhoE(const double *ct, const double *y, double *rho, double *e) {
+// rho and E at offset y from their Taylor coefficients rho_ct[kStride * j],
+// e_ct[kStride * j]: the FMA chains of components 0 and 4 in EvalTaylor.
+template <int P, int kStride>
+__device__ __forceinline__ void EvalTaylorRhoE(const double *rho_ct, const double *e_ct, const double *y, double *rho,
+                                               double *e) {
   constexpr int N = P == 1 ? 4 : 10;
   double t[10];
   t[0] = 1.0; t[1] = y[0]; t[2] = y[1]; t[3] = y[2];
@@ -496,11 +510,11 @@ __device__ __forceinline__ void EvalTaylorRhoE(const double *ct, const double *y
     t[4] = MulRn(y[0], y[0]); t[5] = MulRn(y[0], y[1]); t[6] = MulRn(y[0], y[2]);
     t[7] = MulRn(y[1], y[1]); t[8] = MulRn(y[1], y[2]); t[9] = MulRn(y[2], y[2]);
   }
-  double a0 = ct[0], a4 = ct[4];
+  double a0 = rho_ct[0], a4 = e_ct[0];
 #pragma unroll
   for (int j = 1; j < N; ++j) {
-    a0 = __fma_rn(ct[5 * j], t[j], a0);
-    a4 = __fma_rn(ct[5 * j + 4], t[j], a4);
+    a0 = __fma_rn(rho_ct[kStride * j], t[j], a0);
+    a4 = __fma_rn(e_ct[kStride * j], t[j], a4);
   }
   *rho = a0;
   *e = a4;
@@ -521,7 +535,6 @@ template <bool kHex, int P>
 __global__ __launch_bounds__(256) void DgWenoFlagsKernel(DgLimiterArgs a) {
   constexpr int NA = kHex ? 6 : 4;  // neighbours at most
   constexpr int L = FlagLanes(kHex);
-  constexpr int N = P == 1 ? 4 : 10, PACKED = N * (N + 1) / 2;
   const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t c = w / L;
   const int i = (int)(w & (L - 1));
@@ -529,19 +542,20 @@ __global__ __launch_bounds__(256) void DgWenoFlagsKernel(DgLimiterArgs a) {
   const bool all = a.mode == 2 || a.mode == 4;
   int na = 0;
   double jump0 = 0.0, jump4 = 0.0, abs0 = 0.0, abs4 = 0.0;  // neighbour i's terms
+  const double *rc = a.flag_rec + (on ? c : 0) * kDgFlagRecord;
   if (on && !all) {
     na = a.n_adj[c];
     if (i < na) {
       const int64_t o = a.adj[c * 6 + i];
-      const double *gc = a.geo + c * kDgGeoStride, *go = a.geo + o * kDgGeoStride;
-      const double y[3] = { gc[11] - go[11], gc[12] - go[12], gc[13] - go[13] };
+      const double *ro = a.flag_rec + o * kDgFlagRecord;
+      const double y[3] = { rc[kDgFlagCentre] - ro[kDgFlagCentre], rc[kDgFlagCentre + 1] - ro[kDgFlagCentre + 1],
+                            rc[kDgFlagCentre + 2] - ro[kDgFlagCentre + 2] };
       double rho, e;
-      EvalTaylorRhoE<P>(a.ct + o * 5 * N, y, &rho, &e);
-      const double so = a.basis[o * PACKED];
-      jump0 = fabs(a.ct[c * 5 * N] - rho);  // u(centre) = Ct T(0)
-      jump4 = fabs(a.ct[c * 5 * N + 4] - e);
-      abs0 = fabs(a.u[o * 5 * N] * so);
-      abs4 = fabs(a.u[o * 5 * N + 4] * so);
+      EvalTaylorRhoE<P, 1>(ro, ro + 10, y, &rho, &e);
+      jump0 = fabs(rc[0] - rho);  // u(centre) = Ct T(0)
+      jump4 = fabs(rc[10] - e);
+      abs0 = fabs(ro[kDgFlagAverage]);
+      abs4 = fabs(ro[kDgFlagAverage + 1]);
     }
   }
   const int lane = (int)(threadIdx.x & 63), first = lane & ~(L - 1);
@@ -562,14 +576,12 @@ __global__ __launch_bounds__(256) void DgWenoFlagsKernel(DgLimiterArgs a) {
     if (all) {
       rough = true;
     } else {
-      const double *gc = a.geo + c * kDgGeoStride;
-      const double s00 = a.basis[c * PACKED];
-      double max_abs0 = fmax(1e-9, fabs(a.u[c * 5 * N] * s00)), max_abs4 = fmax(1e-9, fabs(a.u[c * 5 * N + 4] * s00));
+      double max_abs0 = fmax(1e-9, fabs(rc[kDgFlagAverage])), max_abs4 = fmax(1e-9, fabs(rc[kDgFlagAverage + 1]));
       if (na > 0) {
         max_abs0 = fmax(max_abs0, max0);
         max_abs4 = fmax(max_abs4, max4);
       }
-      double divisor = pow(gc[14], (P + 1.0) / 2.0 / 3);
+      double divisor = pow(rc[kDgFlagVolume], (P + 1.0) / 2.0 / 3);
       divisor *= na;
       const double reference = P < 3 ? 1.0 : 3.0;
       rough = sum0 / max_abs0 / divisor > reference || sum4 / max_abs4 / divisor > reference;
@@ -944,15 +956,17 @@ hipError_t LaunchStageR(int riemann, const DgModalArgs &a, hipStream_t stream) {
 }  // namespace
 
 hipError_t LaunchDgTaylorCoefficients(int degree, const double *us, const double *basis, int64_t n_cells,
-                                      double *ct, hipStream_t stream) {
+                                      double *ct, double *flag_rec, hipStream_t stream) {
   const int n = degree == 1 ? 4 : 10;
   const int64_t blocks = (n_cells * 5 * n + 255) / 256;
   if (blocks < 1) return hipSuccess;
   if (blocks > 0x7fffffff) return hipErrorInvalidValue;
   if (degree == 1) {
-    hipLaunchKernelGGL(DgTaylorKernel<1>, dim3((unsigned)blocks), dim3(256), 0, stream, us, basis, n_cells, ct);
+    hipLaunchKernelGGL(DgTaylorKernel<1>, dim3((unsigned)blocks), dim3(256), 0, stream, us, basis, n_cells, ct,
+                       flag_rec);
   } else if (degree == 2) {
-    hipLaunchKernelGGL(DgTaylorKernel<2>, dim3((unsigned)blocks), dim3(256), 0, stream, us, basis, n_cells, ct);
+    hipLaunchKernelGGL(DgTaylorKernel<2>, dim3((unsigned)blocks), dim3(256), 0, stream, us, basis, n_cells, ct,
+                       flag_rec);
   } else {
     return hipErrorInvalidValue;
   }

@@ -24,6 +24,14 @@ constexpr int kDgMaxFacePoints = 9;     // 3^2 Gauss points (quadrangles, p = 2)
 constexpr int kDgMaxOrient = 8;         // dihedral group of the square
 constexpr int kDgGeoStride = 16;        // J rows (9), |det J| (1), length (1), centre (3), volume (1), pad
 constexpr int kDgFaceStride = 8;        // n (3), t (3), area factor (1), pad
+// Per-cell record that the WENO troubled-cell detector reads (two 128-byte
+// lines instead of the scattered coefficient, state, basis and geometry
+// lines): rho Taylor coefficients [0, N), E Taylor coefficients [10, 10 + N),
+// centre [20, 23), volume [23], rho and E average terms u[0][k] * S[0][0]
+// [24], [25].  The centre and volume are set once; the Taylor kernel and the
+// stage pass rewrite the rest for every state they produce.
+constexpr int kDgFlagRecord = 32;
+constexpr int kDgFlagCentre = 20, kDgFlagVolume = 23, kDgFlagAverage = 24;
 
 // Reference-element tables of one (cell type, degree); lives in device memory.
 struct DgTables {
@@ -54,6 +62,7 @@ struct DgModalArgs {
   // If set (with ct_out), the time-step reduction of LaunchDgTimeStep over the
   // state this pass writes is folded into *dt_min_bits (same arithmetic).
   unsigned long long *dt_min_bits;
+  double *flag_rec;       // [cell][kDgFlagRecord] for the WENO detector, or null (no limiter)
   // The pass runs as two launches: w Fhat once per face point into fhat
   // [face][point][5], then the cells read it.  face_cells [face][3]: holder,
   // sharer or -(1 + boundary tag), holder face | sharer face << 3 |
@@ -65,7 +74,7 @@ struct DgModalArgs {
 
 // Ct = C S for every cell of `us` into `ct`.
 hipError_t LaunchDgTaylorCoefficients(int degree, const double *us, const double *basis, int64_t n_cells,
-                                      double *ct, hipStream_t stream);
+                                      double *ct, double *flag_rec, hipStream_t stream);
 // One residual / RK-stage pass (reads a.ct, which must hold the Taylor
 // coefficients of a.us, and writes those of a.out to a.ct_out if set).
 hipError_t LaunchDgModalStage(int cell_type, int degree, int riemann, const DgModalArgs &a, hipStream_t stream);
@@ -81,6 +90,7 @@ struct DgLimiterArgs {
   const int32_t *n_adj;   // [cell]
   const double *faces;    // face records, as DgModalArgs
   const DgTables *tables;
+  const double *flag_rec; // [cell][kDgFlagRecord] of the state u
   uint8_t *flags;         // [cell]
   int32_t *troubled;      // [cell] list of the flagged cells, in no particular order
   int32_t *n_troubled;    // its length
