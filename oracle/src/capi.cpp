@@ -168,6 +168,9 @@ int oracle_fr_create(int64_t n_nodes, double const *coords, int64_t n_cells, int
   });
 }
 void oracle_fr_destroy(void *h) { delete static_cast<FrHandle *>(h); }
+// Plants a known mistake in the FR restatement (FrScheme::mutation), so that
+// tests can show which pins catch it.
+void oracle_fr_set_mutation(void *h, int mutation) { static_cast<FrHandle *>(h)->scheme->mutation = mutation; }
 int64_t oracle_fr_n_faces(void *h, int64_t *n_interior) {
   auto *p = static_cast<FrHandle *>(h);
   *n_interior = p->mesh.n_interior;

@@ -56,6 +56,10 @@ struct FrScheme {
   double dmat[8][8];                         // dmat[i][m] = l_m'(x_i)
   double lend[2][8];                         // l_m(-1), l_m(+1)
   double gprime[2][8];                       // g_L'(x_m), g_R'(x_m)
+  // Planted mistakes for the tests that show a pin catches them (0 = none):
+  // 1 lifts every flux gap onto the transposed line (b, a); 2 takes the
+  // sharer's trace from its transposed line.
+  int mutation = 0;
 
   FrScheme(Mesh const *m, double c_next, int riemann_kind, std::vector<BcEntry> bc)
       : mesh(m), q(m->q), n(m->q * m->q * m->q), vincent(m->q - 1, c_next),
@@ -188,6 +192,7 @@ struct FrScheme {
     };
     auto correct = [&](int64_t c, int i_face, int a, int b, Val5 const &gap) {
       int ijk[8];
+      if (mutation == 1) std::swap(a, b);
       FaceLine(q, i_face, a, b, ijk);
       double const *g = gprime[kFaceSign[i_face] > 0];
       for (int m = 0; m < q; ++m)
@@ -206,7 +211,8 @@ struct FrScheme {
         auto const &ix = fp_index[f * nfp + p];
         Val5 uh, fh, us, fs;
         trace(face.holder, face.holder_face, ix[0], ix[1], &uh, &fh);
-        trace(face.sharer, face.sharer_face, ix[2], ix[3], &us, &fs);
+        if (mutation == 2) trace(face.sharer, face.sharer_face, ix[3], ix[2], &us, &fs);
+        else trace(face.sharer, face.sharer_face, ix[2], ix[3], &us, &fs);
         Val5 fu = riemanns[f * nfp + p].GetFluxUpwind(face.frames[p], uh, us);
         double sh = face_scale(face.holder, face.holder_face, ix[0], ix[1]);
         double ss = face_scale(face.sharer, face.sharer_face, ix[2], ix[3]);

@@ -201,6 +201,12 @@ class FrOracle:
         _check(load().oracle_fr_residual(self.h, _ptr(u, _dp), _ptr(r, _dp)))
         return r
 
+    def set_mutation(self, mutation: int):
+        """Plants a known mistake in the restatement (0 = none); see FrScheme::mutation."""
+        lib = load()
+        lib.oracle_fr_set_mutation.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        lib.oracle_fr_set_mutation(self.h, int(mutation))
+
     def rk3(self, u, dt, n_steps=1):
         u = f64(u).copy()
         _check(load().oracle_fr_rk3(self.h, _ptr(u, _dp), float(dt), int(n_steps)))

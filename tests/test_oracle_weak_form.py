@@ -1,0 +1,158 @@
+"""CPU pins of the hexahedral FR oracle by an independently written scheme.
+
+oracle/nodal_dg.py states collocated nodal DG in weak form (dg::General
+arithmetic of spatial/dg/general.hpp:58-89 with a Lagrange basis and the
+diagonal mass of dg/lobatto.hpp:143-158) from the mesh arrays alone.  Huynh
+(2007) shows that flux reconstruction with the DG correction on
+Gauss-Legendre points, and with the lumped-Lobatto correction on
+Gauss-Lobatto points, equal that scheme.  So the FR restatement of
+oracle/src/fr.hpp (fr/general.hpp, fr/lobatto.hpp) must reproduce it to
+rounding on any state, including states with jumps at every face, where the
+face-correction term of FR is non-zero.
+
+The planted-mistake tests show the pin is sharp: a flux gap lifted onto the
+transposed line, a sharer trace taken from the transposed line, or another
+correction function each move the residual far outside the tolerance.
+
+A grid-convergence test checks the order of accuracy of the FR oracle on a
+smooth periodic density wave (error ratio about 2^(p+1) per halving of h).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from pvcriemannsolvers_amd import meshgen
+from test_boundary_cpu import rotate_cells
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import nodal_dg  # noqa: E402
+
+GAMMA = 1.4
+SMART, WALL, OUTLET, INLET = 0, 1, 2, 3
+HLLC, ROE = 1, 2
+LEFT = [1.0, 0.1, -0.05, 0.02, 2.6]
+RIGHT = [0.4, -0.1, 0.0, 0.05, 0.9]
+
+
+def cons(rho, vel, p):
+    u = np.zeros(np.shape(rho) + (5,))
+    u[..., 0] = rho
+    for d in range(3):
+        u[..., 1 + d] = rho * vel[..., d]
+    u[..., 4] = p / (GAMMA - 1) + 0.5 * rho * np.sum(vel ** 2, axis=-1)
+    return u
+
+
+def rough_state(x, seed):
+    """Smooth fields plus a per-node random part: every face carries a jump."""
+    rng = np.random.default_rng(seed)
+    shape = x.shape[:-1]
+    rho = 1.0 + 0.2 * np.sin(2 * np.pi * x[..., 0]) * np.cos(np.pi * x[..., 1]) + 0.05 * rng.random(shape)
+    vel = np.stack([0.3 + 0.1 * np.cos(2 * np.pi * x[..., 2]), -0.2 + 0.05 * rng.random(shape),
+                    0.1 * np.sin(2 * np.pi * x[..., 0])], axis=-1)
+    p = 1.0 + 0.1 * np.cos(2 * np.pi * x[..., 1]) + 0.05 * rng.random(shape)
+    return cons(rho, vel, p)
+
+
+def sheared(mesh):
+    a = np.array([[1.0, 0.25, 0.1], [0.05, 0.9, 0.3], [0.1, 0.0, 1.1]])
+    return dict(mesh, coords=np.ascontiguousarray(mesh["coords"] @ a.T + [0.2, -0.1, 0.3]))
+
+
+BOX_BCS = [(SMART, LEFT), (SMART, RIGHT), (WALL, [0.0] * 5), (OUTLET, [0.0] * 5), (INLET, LEFT), (WALL, [0.0] * 5)]
+MESHES = {
+    "one_cell": (lambda: meshgen.box_hex(1, 1, 1, hi=(0.5, 0.4, 0.3)), BOX_BCS),
+    "two_cells": (lambda: meshgen.box_hex(2, 1, 1, hi=(0.6, 0.3, 0.3)), BOX_BCS),
+    "sheared_rotated": (lambda: sheared(rotate_cells(meshgen.box_hex(2, 2, 1, shuffle_seed=3), 7)), BOX_BCS),
+    "periodic_rotated": (lambda: rotate_cells(meshgen.box_hex(3, 2, 2, periodic=(True,) * 3, shuffle_seed=4), 8),
+                         []),
+}
+
+
+def upwind(kind):
+    return lambda frame, ul, ur: ol.riemann_flux(kind, frame, ul, ur)
+
+
+def compare(mesh, bcs, q, riemann, lobatto, c_next, seed, mutation=0):
+    fr = ol.FrOracle(mesh, q=q, riemann=riemann, lobatto=lobatto, c_next=c_next, bcs=bcs)
+    if mutation:
+        fr.set_mutation(mutation)
+    weak = nodal_dg.NodalDgWeak(mesh, q, bcs=bcs, lobatto=lobatto)
+    u = rough_state(weak.x_nodes, seed).reshape(-1)
+    r_fr = fr.residual(u)
+    r_weak = weak.residual(u, upwind(riemann))
+    return np.abs(r_fr - r_weak).max() / np.abs(r_weak).max(), fr, weak
+
+
+@pytest.mark.parametrize("name", sorted(MESHES))
+def test_shared_layout_node_coordinates(name):
+    make, bcs = MESHES[name]
+    mesh = make()
+    fr = ol.FrOracle(mesh, q=3, riemann=HLLC, bcs=bcs)
+    weak = nodal_dg.NodalDgWeak(mesh, 3, bcs=bcs)
+    np.testing.assert_allclose(fr.node_coords(), weak.x_nodes, rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("name", sorted(MESHES))
+@pytest.mark.parametrize("riemann", [HLLC, ROE])
+@pytest.mark.parametrize("q", [2, 3, 4])
+def test_gauss_legendre_fr_dg_equals_nodal_dg(name, riemann, q):
+    make, bcs = MESHES[name]
+    err, _, _ = compare(make(), bcs, q, riemann, lobatto=False, c_next=1.0, seed=q + 10 * riemann)
+    assert err < 1e-12, err
+
+
+@pytest.mark.parametrize("name", sorted(MESHES))
+@pytest.mark.parametrize("q", [2, 3, 4, 5])
+def test_lobatto_lumped_fr_equals_dgsem(name, q):
+    """fr::Lobatto (c_next = HuynhLumpingLobatto(p) = p / (2p + 1)) is DGSEM, dg::Lobatto's arithmetic."""
+    make, bcs = MESHES[name]
+    p = q - 1
+    err, _, _ = compare(make(), bcs, q, HLLC, lobatto=True, c_next=p / (2 * p + 1), seed=q)
+    assert err < 1e-12, err
+
+
+@pytest.mark.parametrize("mutation", [1, 2])
+@pytest.mark.parametrize("name", ["two_cells", "periodic_rotated"])
+def test_planted_transposed_line_is_caught(name, mutation):
+    make, bcs = MESHES[name]
+    err, _, _ = compare(make(), bcs, 3, HLLC, lobatto=False, c_next=1.0, seed=1, mutation=mutation)
+    assert err > 1e-4, err
+
+
+@pytest.mark.parametrize("lobatto", [False, True])
+def test_planted_correction_function_is_caught(lobatto):
+    make, bcs = MESHES["two_cells"]
+    q = 4
+    wrong = 0.5 if not lobatto else 1.0  # right would be 1 (DG) and 3/7 (lumped Lobatto)
+    err, _, _ = compare(make(), bcs, q, HLLC, lobatto=lobatto, c_next=wrong, seed=2)
+    assert err > 1e-4, err
+
+
+def wave_error(n, q, t_end=0.05):
+    """L2 error of rho after advecting 1 + 0.2 sin(2 pi (x + y + z)) with u = (1, 0.5, 0.25), p = 1."""
+    mesh = meshgen.box_hex(n, n, n, periodic=(True,) * 3)
+    o = ol.FrOracle(mesh, q=q, riemann=ROE)
+    vel = np.array([1.0, 0.5, 0.25])
+
+    def exact(x, t):
+        rho = 1 + 0.2 * np.sin(2 * np.pi * ((x - vel * t) @ np.ones(3)))
+        return cons(rho, np.broadcast_to(vel, x.shape), np.ones(x.shape[:-1]))
+
+    x = o.node_coords()
+    u = exact(x, 0.0).reshape(-1)
+    steps = int(np.ceil(t_end / (0.05 / n / q)))
+    u = o.rk3(u, t_end / steps, steps)
+    err = (u.reshape(x.shape[:-1] + (5,))[..., 0] - exact(x, t_end)[..., 0]) ** 2
+    return np.sqrt(np.sum(o.node_weights() * err))
+
+
+@pytest.mark.parametrize("q,n", [(2, 8), (3, 6), (4, 4)])
+def test_order_of_accuracy(q, n):
+    """Measured on the oracle: 1.68, 2.65 and 3.64 at these resolutions, rising towards p + 1 = q."""
+    e1, e2 = wave_error(n, q), wave_error(2 * n, q)
+    order = np.log2(e1 / e2)
+    assert order > q - 0.5, (e1, e2, order)
