@@ -86,11 +86,18 @@ extern "C" {
 #define MCFD_LIMITER_WENO_EIGEN 3
 #define MCFD_LIMITER_WENO_EIGEN_ALL 4
 
-/* ---- boundary conditions, spatial/fem.hpp:196-222 (constant states) */
+/* ---- boundary conditions, spatial/fem.hpp:196-222.  The `state` of an
+ * mcfd_bc_desc is the tag's constant given value; mcfd_set_boundary_function
+ * replaces it by a function of the point and the time.
+ * SUBSONIC_INLET: state = (p_total, u_cos, v_cos, w_cos, T_total), the total
+ *   conditions and global flow direction of riemann/rotated/euler.hpp:181-246;
+ * SUBSONIC_OUTLET: state[4] = the back pressure (euler.hpp:263-280). */
 #define MCFD_BC_SMART 0               /* SetSmartBoundary(name, f) */
 #define MCFD_BC_INVISCID_WALL 1       /* SetInviscidWall(name) */
 #define MCFD_BC_SUPERSONIC_OUTLET 2   /* SetSupersonicOutlet(name) */
 #define MCFD_BC_SUPERSONIC_INLET 3    /* SetSupersonicInlet(name, f) */
+#define MCFD_BC_SUBSONIC_INLET 4      /* SetSubsonicInlet(name, f) */
+#define MCFD_BC_SUBSONIC_OUTLET 5     /* SetSubsonicOutlet(name, f) */
 
 typedef struct mcfd_solver mcfd_solver;
 
@@ -184,10 +191,53 @@ int mcfd_face_count(const mcfd_solver *h, int64_t *n_faces, int64_t *n_interior)
 int mcfd_faces(const mcfd_solver *h, int64_t *holder, int64_t *sharer,
                int32_t *holder_face, int32_t *sharer_face, int32_t *fp_ijk);
 
+/* Boundary values that vary in space and time: the function argument of
+ * SetSmartBoundary / SetSupersonicInlet / SetSubsonicInlet /
+ * SetSubsonicOutlet (spatial/fem.hpp:196-222), which the reference evaluates
+ * at every boundary quadrature point at the system's time whenever it forms a
+ * residual (dg/general.hpp:92-202, fr/general.hpp:435-640).  The library calls
+ * f once per tag with all of the tag's points, states[n][5] = f(xyz[n][3], t),
+ * on the calling thread, whenever a residual is formed at a new time: the
+ * time of mcfd_set_time for mcfd_get_residual_column, and t, t + dt and
+ * t + dt / 2 for the stages of a fused RK3 step (temporal/rk.hpp:31-49).
+ * f = NULL restores the tag's constant state.  mcfd_boundary_points gives the
+ * tag's points in the order f receives them ([n][3]; xyz may be null to query
+ * n). */
+typedef void (*mcfd_bc_function)(int64_t n, const double *xyz, double t, double *states, void *user);
+int mcfd_boundary_points(const mcfd_solver *h, int32_t tag, int64_t *n_points, double *xyz);
+int mcfd_set_boundary_function(mcfd_solver *h, int32_t tag, mcfd_bc_function f, void *user);
+
 /* Troubled-cell flags of the last limiter application (1 = reconstructed),
  * one byte per cell, and their count; limiter/reconstruct.hpp:29-34.  Either
  * output may be null. */
 int mcfd_limiter_flags(mcfd_solver *h, uint8_t *flags, int64_t *n_troubled);
+
+/* ---- face fluxes at independent points
+ * The device code of the stage kernels evaluated on caller-given state
+ * pairs (host arrays), for checking the solvers on known answers:
+ *   kind UPWIND: rotated::Euler<...>::GetFluxUpwind (riemann/rotated/euler.hpp:
+ *     153-159) with the solver `riemann`;
+ *   kind INVISCID_WALL: GetFluxOnInviscidWall (euler.hpp:160-174) of `left`;
+ *   kind SUPERSONIC_OUTLET: GetFluxOnSupersonicOutlet (euler.hpp:144-151) of `left`;
+ *   kind SUBSONIC_INLET / SUBSONIC_OUTLET: GetFluxOnSubsonicInlet / Outlet
+ *     (euler.hpp:181-280) of `left` with the given values in `right`.
+ * form FR: the FR stage kernels' evaluation (Roe with one reciprocal of rho per
+ * side); form DG: the modal-DG face pass's.  frames[n][9] holds the rows
+ * (n, t, b) of each point's frame; left, right [n][5] conservative states
+ * (right may be null for INVISCID_WALL and SUPERSONIC_OUTLET); flux [n][5] out.  exact_star[n]
+ * (in/out, may be null = fresh solvers) is each point's exact-solver state:
+ * 1 if the star velocity of its last ordinary solve was positive
+ * (riemann/euler/exact.hpp:29,116), which the vacuum branches reuse for the
+ * tangential velocities (exact.hpp:49-57). */
+#define MCFD_FLUX_UPWIND 0
+#define MCFD_FLUX_INVISCID_WALL 1
+#define MCFD_FLUX_SUPERSONIC_OUTLET 2
+#define MCFD_FLUX_SUBSONIC_INLET 3
+#define MCFD_FLUX_SUBSONIC_OUTLET 4
+#define MCFD_FLUX_FORM_FR 0
+#define MCFD_FLUX_FORM_DG 1
+int mcfd_face_fluxes(int32_t riemann, int32_t form, int32_t kind, int64_t n, const double *frames,
+                     const double *left, const double *right, uint8_t *exact_star, double *flux);
 
 /* Stream synchronisation and per-kernel timing of n RK3 steps with HIP
  * events on the handle's stream: total ms, summed ms of the stage kernels,
@@ -229,6 +279,11 @@ int mcfd_batch_set_time_steps(mcfd_batch *b, const double *dt_per_case);
  * graph.  tiles[g] = 16 * Q + variant for each degree group in ascending
  * degree; *mode = 1 persistent, 0 per group (either output may be null). */
 int mcfd_batch_autotune(mcfd_batch *b, int32_t repeats, int32_t *tiles, int32_t *mode);
+/* Fixes the launch form instead of the autotuner's choice and re-captures
+ * the step graph: mode 1 = one persistent launch per stage, 0 = one launch
+ * per degree group; tiles[g] (0..2, degree groups in ascending degree) is the
+ * tile variant of the persistent launch, or null to keep the current ones. */
+int mcfd_batch_set_mode(mcfd_batch *b, int32_t mode, const int32_t *tiles);
 /* n_steps replays of the captured SSP-RK3 step (stream-ordered). */
 int mcfd_batch_rk3_steps(mcfd_batch *b, int32_t n_steps);
 int mcfd_batch_synchronize(mcfd_batch *b);

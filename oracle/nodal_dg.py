@@ -41,7 +41,7 @@ CORNERS = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
                     [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], dtype=np.float64)
 
 # Boundary kinds of the BC table (same meaning as the product's table).
-SMART, WALL, OUTLET, INLET = 0, 1, 2, 3
+SMART, WALL, OUTLET, INLET, SUB_INLET, SUB_OUTLET = 0, 1, 2, 3, 4, 5
 
 
 def rule(q: int, lobatto: bool):
@@ -237,9 +237,11 @@ class NodalDgWeak:
         return vals[0][:, idx[:, 0]] * vals[1][:, idx[:, 1]] * vals[2][:, idx[:, 2]]
 
     # ---- residual
-    def residual(self, u: np.ndarray, upwind) -> np.ndarray:
+    def residual(self, u: np.ndarray, upwind, boundary=None) -> np.ndarray:
         """du/dt in the column layout [cell][node][5].  ``upwind(frame, ul, ur)``
-        is the rotated upwind flux with frame rows (n, t, b), n from ul to ur."""
+        is the rotated upwind flux with frame rows (n, t, b), n from ul to ur;
+        ``boundary(kind, frame, u_in, given)`` the subsonic inlet and outlet
+        fluxes, whose formulas are pinned on their own."""
         nc = len(self.cells)
         u = np.asarray(u, dtype=np.float64).reshape(nc, self.n, 5)
         r = np.zeros_like(u)
@@ -283,6 +285,8 @@ class NodalDgWeak:
                     fl = euler_flux(uc[p]) @ n
                 elif kind == INLET:
                     fl = euler_flux(given) @ n
+                elif kind in (SUB_INLET, SUB_OUTLET):
+                    fl = boundary(kind, _frame(n), uc[p], given)
                 else:
                     fl = upwind(_frame(n), uc[p], given)
                 r[c] -= np.outer(phi[p], fl * wf[p])

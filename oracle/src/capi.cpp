@@ -81,6 +81,29 @@ void Rk3Step(size_t size, double *u, double dt, Residual &&residual) {
   Rk3Step(size, u, dt, residual, [](double *) {});
 }
 
+// Points of the boundary faces, [boundary face][point][3] in the restatement's
+// order, and the given states there ([same][5], or n = 0 for the tags'
+// constant states): the caller evaluates the BC functions f(x, t).
+template <class Handle>
+void BoundaryPoints(Handle *p, double *xyz, int32_t *tags) {
+  size_t i = 0, j = 0;
+  for (size_t f = p->mesh.n_interior; f < p->mesh.faces.size(); ++f)
+    for (auto const &x : p->mesh.faces[f].qp) {
+      for (int d = 0; d < 3; ++d) xyz[i++] = x[d];
+      if (tags) tags[j++] = p->mesh.faces[f].tag;
+    }
+}
+template <class Handle>
+int64_t BoundaryPointCount(Handle *p) {
+  int64_t n = 0;
+  for (size_t f = p->mesh.n_interior; f < p->mesh.faces.size(); ++f) n += p->mesh.faces[f].qp.size();
+  return n;
+}
+template <class Scheme>
+void SetPointStates(Scheme *s, int64_t n, double const *states) {
+  s->point_state.resize(n);
+  for (int64_t i = 0; i < n; ++i) std::memcpy(s->point_state[i].data(), states + 5 * i, 40);
+}
 }  // namespace
 
 extern "C" {
@@ -143,6 +166,32 @@ void oracle_riemann_flux(int kind, double const *frame9, double const *cl, doubl
   Val5 o = re.GetFluxUpwind(f, l, r);
   std::memcpy(flux, o.data(), 40);
 }
+// The fluxes of mcfd_face_fluxes at n points, each with its own solver
+// object whose exact-solver state star_u[i] (exact.hpp:29) is read and
+// written back.  kind 0 upwind, 1 inviscid wall, 2 supersonic outlet,
+// 3 subsonic inlet and 4 subsonic outlet with the given values in `right`.
+void oracle_face_fluxes(int riemann, int kind, int64_t n, double const *frames, double const *left,
+    double const *right, double *star_u, double *flux) {
+  for (int64_t i = 0; i < n; ++i) {
+    Frame f;
+    for (int d = 0; d < 3; ++d) {
+      f.a[d] = frames[9 * i + d]; f.b[d] = frames[9 * i + 3 + d]; f.c[d] = frames[9 * i + 6 + d];
+    }
+    Val5 l, r{};
+    std::memcpy(l.data(), left + 5 * i, 40);
+    if (right) std::memcpy(r.data(), right + 5 * i, 40);
+    RotatedEuler re;
+    re.kind = RiemannKind(riemann);
+    if (star_u) re.exact.star_u = star_u[i];
+    Val5 o = kind == 1   ? re.GetFluxOnInviscidWall(f, l)
+             : kind == 2 ? re.GetFluxOfState(f, l)
+             : kind == 3 ? re.GetFluxOnSubsonicInlet(f, l, r)
+             : kind == 4 ? re.GetFluxOnSubsonicOutlet(f, l, r)
+                         : re.GetFluxUpwind(f, l, r);
+    if (star_u) star_u[i] = re.exact.star_u;
+    std::memcpy(flux + 5 * i, o.data(), 40);
+  }
+}
 void oracle_wall_flux(double const *frame9, double const *c, double *flux) {
   Frame f;
   for (int d = 0; d < 3; ++d) { f.a[d] = frame9[d]; f.b[d] = frame9[3 + d]; f.c[d] = frame9[6 + d]; }
@@ -168,6 +217,15 @@ int oracle_fr_create(int64_t n_nodes, double const *coords, int64_t n_cells, int
   });
 }
 void oracle_fr_destroy(void *h) { delete static_cast<FrHandle *>(h); }
+int64_t oracle_fr_boundary_points(void *h, double *xyz, int32_t *tags) {
+  auto *p = static_cast<FrHandle *>(h);
+  if (xyz) BoundaryPoints(p, xyz, tags);
+  return BoundaryPointCount(p);
+}
+void oracle_fr_set_boundary_states(void *h, int64_t n, double const *states) {
+  SetPointStates(static_cast<FrHandle *>(h)->scheme.get(), n, states);
+}
+
 // Plants a known mistake in the FR restatement (FrScheme::mutation), so that
 // tests can show which pins catch it.
 void oracle_fr_set_mutation(void *h, int mutation) { static_cast<FrHandle *>(h)->scheme->mutation = mutation; }
@@ -300,6 +358,14 @@ void oracle_dg_faces(void *h, int64_t *holder, int64_t *sharer, int32_t *holder_
       }
     }
   }
+}
+int64_t oracle_dg_boundary_points(void *h, double *xyz, int32_t *tags) {
+  auto *p = static_cast<DgHandle *>(h);
+  if (xyz) BoundaryPoints(p, xyz, tags);
+  return BoundaryPointCount(p);
+}
+void oracle_dg_set_boundary_states(void *h, int64_t n, double const *states) {
+  SetPointStates(static_cast<DgHandle *>(h)->scheme.get(), n, states);
 }
 int oracle_dg_residual(void *h, double const *u, double *r) {
   return Guard([&] { static_cast<DgHandle *>(h)->scheme->Residual(u, r); });

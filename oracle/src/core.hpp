@@ -505,6 +505,45 @@ struct RotatedEuler {
   Val5 GetFluxOnSmartBoundary(Frame const &f, Val5 const &ci, Val5 const &ce) {
     return GetFluxUpwind(f, ci, ce);
   }
+  // euler.hpp:181-246, the default branch (no boundary Riemann problem):
+  // given = (p_total, u_cos, v_cos, w_cos, T_total).
+  Val5 GetFluxOnSubsonicInlet(Frame const &f, Val5 const &ci, Val5 const &given) {
+    constexpr double kR = 287.05, kGm1 = 0.4, kGamma = 1.4;  // types.hpp:206-208
+    Val5 pi = ConservativeToPrimitive(ci);
+    double c_interior = (pi[0] > 0 && pi[4] > 0) ? std::sqrt(kGamma * pi[4] / pi[0]) : 0;
+    GlobalToNormal(f, &pi);
+    double riemann_interior = -pi[1] - c_interior * (2 / kGm1);
+    double u_cos = given[1], v_cos = given[2], w_cos = given[3], t_total = given[4];
+    double a = f.a[0] * u_cos + f.a[1] * v_cos + f.a[2] * w_cos;
+    double a2 = a * a, gamma_times_r = kGamma * kR;
+    auto larger = [](double qa, double b_half, double qc) {
+      double sqrt_delta = std::sqrt(b_half * b_half - qa * qc);
+      double x1 = (-b_half + sqrt_delta) / qa, x2 = (-b_half - sqrt_delta) / qa;
+      return std::max(x1, x2);
+    };
+    double c_boundary = larger(a2 + 2 / kGm1, riemann_interior,
+        kGm1 / 2 * (riemann_interior * riemann_interior) - a2 * gamma_times_r * t_total);
+    double t_boundary = c_boundary * c_boundary / gamma_times_r;
+    double mach = std::sqrt((t_total / t_boundary - 1) / (kGm1 / 2));  // types.hpp:273-280
+    double p_boundary = given[0] / std::pow(1 + kGm1 / 2 * mach * mach, kGamma / kGm1);  // :268-272
+    double speed = mach * c_boundary;
+    Val5 pb{ p_boundary / kR / t_boundary, u_cos * speed, v_cos * speed, w_cos * speed, p_boundary };
+    GlobalToNormal(f, &pb);
+    Val5 flux = PrimitiveToFlux(pb);
+    NormalToGlobal(f, &flux);
+    return flux;
+  }
+  // euler.hpp:263-280, the default branch: given[4] = back pressure.
+  Val5 GetFluxOnSubsonicOutlet(Frame const &f, Val5 const &ci, Val5 const &given) {
+    Val5 pb = ConservativeToPrimitive(ci);
+    double rt_interior = pb[4] / pb[0];
+    pb[4] = given[4];
+    pb[0] = pb[4] / rt_interior;
+    GlobalToNormal(f, &pb);
+    Val5 flux = PrimitiveToFlux(pb);
+    NormalToGlobal(f, &flux);
+    return flux;
+  }
 };
 
 }  // namespace oracle

@@ -103,8 +103,12 @@ struct ModalBasis {  // basis::OrthoNormal
 struct DgScheme {
   Mesh const *mesh;
   int p, n;
-  RotatedEuler riemann;
+  RotatedEuler riemann;                 // the solvers that keep no state
+  std::vector<RotatedEuler> riemanns;   // [face][point], fem.hpp:95-115 (the exact solver's star_u)
   std::vector<BcEntry> bcs;
+  // Given states per boundary face point, [boundary face][point] (the BC
+  // functions f(x, t) at the residual's time); empty: each tag's constant.
+  std::vector<Val5> point_state;
   std::vector<ModalBasis> basis;
   // Cell::adj_cells_ (mesh/part.hpp:962-963): neighbours in interior-face order.
   std::vector<std::vector<int64_t>> adj;
@@ -118,10 +122,15 @@ struct DgScheme {
   double weno_w0 = 0.001, weno_eps = 1e-6;
   std::vector<uint8_t> flags;  // of the last limiter application
 
+  // index of point q of face fi in [face][point] order
+  size_t FacePoint(size_t fi, size_t q) const { return fi * mesh->faces[0].qp.size() + q; }
+
   DgScheme(Mesh const *m, int degree, int riemann_kind, std::vector<BcEntry> bc)
       : mesh(m), p(degree), n(TaylorSize(degree)), bcs(std::move(bc)) {
     if (degree < 1 || degree > 2) throw std::runtime_error("modal DG: degree must be 1 or 2");
     riemann.kind = RiemannKind(riemann_kind);
+    for (auto const &face : mesh->faces)
+      for (size_t q = 0; q < face.qp.size(); ++q) riemanns.push_back(riemann);
     basis.resize(mesh->cells.size());
     for (size_t c = 0; c < mesh->cells.size(); ++c) {
       auto const &cell = mesh->cells[c];
@@ -182,16 +191,15 @@ struct DgScheme {
         Vec3 xs = add(face.qp[q], face.shift);
         Val5 uh = Value(face.holder, u, face.qp[q]);
         Val5 us = Value(face.sharer, u, xs);
-        Val5 flux = riemann.GetFluxUpwind(face.frames[q], uh, us);
+        Val5 flux = riemanns[FacePoint(fi, q)].GetFluxUpwind(face.frames[q], uh, us);
         for (auto &x : flux) x *= -face.qw[q];
         AddRankOne(face.holder, flux, face.qp[q], r);
         for (auto &x : flux) x = -x;
         AddRankOne(face.sharer, flux, xs, r);
       }
     }
-    // fem.hpp:261-323: supersonic inlets, outlets, (subsonic: none), walls, smart.
-    const int family_order[] = { kBcSupersonicInlet, kBcSupersonicOutlet, kBcInviscidWall, kBcSmart };
-    for (int family : family_order) {
+    // fem.hpp:261-323: supersonic inlets and outlets, subsonic inlets and outlets, walls, smart.
+    for (int family : kBcFamilyOrder) {
       for (size_t fi = mesh->n_interior; fi < mesh->faces.size(); ++fi) {
         auto const &face = mesh->faces[fi];
         if (face.tag < 0 || face.tag >= (int)bcs.size()) throw std::runtime_error("boundary tag without BC");
@@ -199,13 +207,18 @@ struct DgScheme {
         for (size_t q = 0; q < face.qp.size(); ++q) {
           Val5 flux;
           auto const &frame = face.frames[q];
+          auto &rs = riemanns[FacePoint(fi, q)];
+          Val5 const &given = point_state.empty()
+              ? bcs[face.tag].state : point_state[FacePoint(fi, q) - FacePoint(mesh->n_interior, 0)];
           if (family == kBcSupersonicInlet) {
-            flux = riemann.GetFluxOfState(frame, bcs[face.tag].state);
+            flux = rs.GetFluxOfState(frame, given);
           } else {
             Val5 uh = Value(face.holder, u, face.qp[q]);
-            if (family == kBcSupersonicOutlet) flux = riemann.GetFluxOfState(frame, uh);
-            else if (family == kBcInviscidWall) flux = riemann.GetFluxOnInviscidWall(frame, uh);
-            else flux = riemann.GetFluxOnSmartBoundary(frame, uh, bcs[face.tag].state);
+            if (family == kBcSupersonicOutlet) flux = rs.GetFluxOfState(frame, uh);
+            else if (family == kBcInviscidWall) flux = rs.GetFluxOnInviscidWall(frame, uh);
+            else if (family == kBcSubsonicInlet) flux = rs.GetFluxOnSubsonicInlet(frame, uh, given);
+            else if (family == kBcSubsonicOutlet) flux = rs.GetFluxOnSubsonicOutlet(frame, uh, given);
+            else flux = rs.GetFluxOnSmartBoundary(frame, uh, given);
           }
           for (auto &x : flux) x *= -face.qw[q];
           AddRankOne(face.holder, flux, face.qp[q], r);

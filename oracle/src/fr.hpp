@@ -19,7 +19,11 @@
 
 namespace oracle {
 
-enum BcKind { kBcSmart = 0, kBcInviscidWall = 1, kBcSupersonicOutlet = 2, kBcSupersonicInlet = 3 };
+enum BcKind { kBcSmart = 0, kBcInviscidWall = 1, kBcSupersonicOutlet = 2, kBcSupersonicInlet = 3,
+              kBcSubsonicInlet = 4, kBcSubsonicOutlet = 5 };
+// fem.hpp:261-323: the order in which the boundary families are added.
+constexpr int kBcFamilyOrder[] = { kBcSupersonicInlet, kBcSupersonicOutlet, kBcSubsonicInlet, kBcSubsonicOutlet,
+                                   kBcInviscidWall, kBcSmart };
 struct BcEntry { int kind; Val5 state; };
 
 // Line (Q solution points normal to a face) of local face i_face through
@@ -60,6 +64,10 @@ struct FrScheme {
   // 1 lifts every flux gap onto the transposed line (b, a); 2 takes the
   // sharer's trace from its transposed line.
   int mutation = 0;
+  // Given states per boundary face point, [boundary face][point] (the BC
+  // functions f(x, t) evaluated by the caller at the time of the residual);
+  // empty: each tag's constant state.
+  std::vector<Val5> point_state;
 
   FrScheme(Mesh const *m, double c_next, int riemann_kind, std::vector<BcEntry> bc)
       : mesh(m), q(m->q), n(m->q * m->q * m->q), vincent(m->q - 1, c_next),
@@ -226,8 +234,7 @@ struct FrScheme {
       }
     }
     // AddFluxOnBoundaries, fem.hpp:260-307 (family order), fr/general.hpp:435-640
-    const int family_order[] = { kBcSupersonicInlet, kBcSupersonicOutlet, kBcInviscidWall, kBcSmart };
-    for (int family : family_order) {
+    for (int family : kBcFamilyOrder) {
       for (size_t tag = 0; tag < bcs.size(); ++tag) {
         if (bcs[tag].kind != family) continue;
         for (size_t f = mesh->n_interior; f < mesh->faces.size(); ++f) {
@@ -238,12 +245,16 @@ struct FrScheme {
             Val5 uh, fh;
             trace(face.holder, face.holder_face, ix[0], ix[1], &uh, &fh);
             auto &rs = riemanns[f * nfp + p];
+            Val5 const &given = point_state.empty() ? bcs[tag].state
+                                                    : point_state[(f - mesh->n_interior) * nfp + p];
             Val5 fu;
             switch (family) {
             case kBcInviscidWall: fu = rs.GetFluxOnInviscidWall(face.frames[p], uh); break;
             case kBcSupersonicOutlet: fu = rs.GetFluxOfState(face.frames[p], uh); break;
-            case kBcSupersonicInlet: fu = rs.GetFluxOfState(face.frames[p], bcs[tag].state); break;
-            default: fu = rs.GetFluxOnSmartBoundary(face.frames[p], uh, bcs[tag].state); break;
+            case kBcSupersonicInlet: fu = rs.GetFluxOfState(face.frames[p], given); break;
+            case kBcSubsonicInlet: fu = rs.GetFluxOnSubsonicInlet(face.frames[p], uh, given); break;
+            case kBcSubsonicOutlet: fu = rs.GetFluxOnSubsonicOutlet(face.frames[p], uh, given); break;
+            default: fu = rs.GetFluxOnSmartBoundary(face.frames[p], uh, given); break;
             }
             double sh = face_scale(face.holder, face.holder_face, ix[0], ix[1]);
             Val5 gh;

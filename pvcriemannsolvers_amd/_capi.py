@@ -22,6 +22,9 @@ PRECISION_FP64, PRECISION_MIXED = 0, 1
 FLAG_HOST_ONLY = 1
 LIMITER_NONE, LIMITER_WENO_FLAGGED, LIMITER_WENO_ALL, LIMITER_WENO_EIGEN, LIMITER_WENO_EIGEN_ALL = 0, 1, 2, 3, 4
 BC_SMART, BC_INVISCID_WALL, BC_SUPERSONIC_OUTLET, BC_SUPERSONIC_INLET = 0, 1, 2, 3
+BC_SUBSONIC_INLET, BC_SUBSONIC_OUTLET = 4, 5
+FLUX_UPWIND, FLUX_INVISCID_WALL, FLUX_SUPERSONIC_OUTLET, FLUX_SUBSONIC_INLET, FLUX_SUBSONIC_OUTLET = 0, 1, 2, 3, 4
+FLUX_FORM_FR, FLUX_FORM_DG = 0, 1
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _i64p = ctypes.POINTER(ctypes.c_int64)
@@ -45,6 +48,9 @@ class BcDesc(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("reserved", ctypes.c_int32), ("state", ctypes.c_double * 5)]
 
 
+# mcfd_bc_function: states[n][5] = f(xyz[n][3], t)
+BC_FUNCTION = ctypes.CFUNCTYPE(None, ctypes.c_int64, _dp, ctypes.c_double, _dp, ctypes.c_void_p)
+
 # Every function declared in include/minicfd.h with its ctypes signature.
 SIGNATURES = {
     "mcfd_create": [ctypes.POINTER(MeshDesc), ctypes.POINTER(SchemeDesc), ctypes.POINTER(BcDesc),
@@ -64,6 +70,10 @@ SIGNATURES = {
     "mcfd_face_count": [ctypes.c_void_p, _i64p, _i64p],
     "mcfd_faces": [ctypes.c_void_p, _i64p, _i64p, _i32p, _i32p, _i32p],
     "mcfd_limiter_flags": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint8), _i64p],
+    "mcfd_boundary_points": [ctypes.c_void_p, ctypes.c_int32, _i64p, _dp],
+    "mcfd_set_boundary_function": [ctypes.c_void_p, ctypes.c_int32, BC_FUNCTION, ctypes.c_void_p],
+    "mcfd_face_fluxes": [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, _dp, _dp, _dp,
+                         ctypes.POINTER(ctypes.c_uint8), _dp],
     "mcfd_synchronize": [ctypes.c_void_p],
     "mcfd_time_rk3": [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32, _dp, _dp, _dp],
     "mcfd_batch_create": [ctypes.c_int32, ctypes.POINTER(MeshDesc), _i32p, ctypes.POINTER(SchemeDesc),
@@ -76,6 +86,7 @@ SIGNATURES = {
     "mcfd_batch_get_time_steps": [ctypes.c_void_p, ctypes.c_double, _dp],
     "mcfd_batch_set_time_steps": [ctypes.c_void_p, _dp],
     "mcfd_batch_autotune": [ctypes.c_void_p, ctypes.c_int32, _i32p, _i32p],
+    "mcfd_batch_set_mode": [ctypes.c_void_p, ctypes.c_int32, _i32p],
     "mcfd_batch_rk3_steps": [ctypes.c_void_p, ctypes.c_int32],
     "mcfd_batch_synchronize": [ctypes.c_void_p],
     "mcfd_batch_time_rk3": [ctypes.c_void_p, ctypes.c_int32, _dp],

@@ -351,6 +351,19 @@ int mcfd_batch_autotune(mcfd_batch *b, int32_t repeats, int32_t *tiles, int32_t 
   });
 }
 
+int mcfd_batch_set_mode(mcfd_batch *b, int32_t mode, const int32_t *tiles) {
+  if (!b) return Fail(MCFD_EINVAL, "null batch");
+  if (mode != 0 && mode != 1) return Fail(MCFD_EINVAL, "mode must be 0 (per degree group) or 1 (persistent)");
+  if (tiles)
+    for (size_t g = 0; g < b->groups.size(); ++g)
+      if (tiles[g] < 0 || tiles[g] > 2) return Fail(MCFD_EINVAL, "tile variant must be 0, 1 or 2");
+  return Guard([&] {
+    b->mode = mode;
+    if (tiles) std::copy(tiles, tiles + b->groups.size(), b->tiles.begin());
+    b->Capture();
+  });
+}
+
 int mcfd_batch_rk3_steps(mcfd_batch *b, int32_t n_steps) {
   if (!b || n_steps < 0) return Fail(MCFD_EINVAL, "bad argument");
   return Guard([&] {

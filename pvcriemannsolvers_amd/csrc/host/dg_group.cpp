@@ -162,7 +162,7 @@ void DgGroup::Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &scheme,
   for (int64_t b = 0; b < mesh.n_bfaces; ++b)
     if (mesh.bface_tags[b] >= n_bcs) throw std::invalid_argument("boundary tag without a BC entry");
   for (auto k : bc_kinds)
-    if (k < 0 || k > 3) throw std::invalid_argument("unknown BC kind");
+    if (k < 0 || k > 5) throw std::invalid_argument("unknown BC kind");
   h_bc_kind = bc_kinds;
   h_bc_state = bc_states;
   if (h_bc_kind.empty()) {
@@ -393,6 +393,10 @@ void DgGroup::Upload(hipStream_t stream) {
   face_rec.Upload(h_faces, stream);
   face_cells.Upload(h_face_cells, stream);
   fhat.Allocate(faces.size() * nfp * 5);
+  if (riemann == MCFD_RIEMANN_EXACT) {
+    exact_star.Allocate(faces.size() * nfp);
+    Check(hipMemsetAsync(exact_star.ptr, 0, faces.size() * nfp, stream), "memset");
+  }
   bc_state.Upload(h_bc_state, stream);
   finfo.Upload(h_finfo, stream);
   bc_kind.Upload(h_bc_kind, stream);
@@ -450,11 +454,30 @@ void DgGroup::Stage(int stage, double dt, hipStream_t stream, unsigned long long
   a.n_cells = n_cells; a.dt = dt; a.stage = stage;
   a.dt_min_bits = stage > 0 && !limiter ? dt_min_bits : nullptr;
   a.fhat = fhat.ptr;
+  a.exact_star = exact_star.ptr;
+  a.bc_point_state = bc_point_state;
+  a.n_interior = n_interior;
   a.flag_rec = limiter && stage > 0 ? flag_rec.ptr : nullptr;
   a.face_cells = face_cells.ptr;
   a.n_faces = (int64_t)faces.size();
   Check(LaunchDgModalStage(cell_type, degree, riemann, a, stream), "DG stage launch");
   if (stage > 0) Limit(a.out, a.ct_out, stream);  // spatial/with_limiter.hpp:56-59 after each SetSolutionColumn
+}
+
+void DgGroup::BoundaryPoints(std::vector<int32_t> *tags, std::vector<double> *xyz) const {
+  const int64_t nb = (int64_t)faces.size() - n_interior;
+  tags->assign(nb, 0);
+  xyz->assign(nb * nfp * 3, 0.0);
+  for (int64_t b = 0; b < nb; ++b) {
+    const auto &face = faces[n_interior + b];
+    (*tags)[b] = face.tag;
+    const double *g = h_geo.data() + face.holder * kDgGeoStride;  // J rows, then the centre at [11..13]
+    for (int q = 0; q < nfp; ++q) {
+      const double *e = tables.face_eta[face.holder_face][0][q];
+      for (int d = 0; d < 3; ++d)
+        (*xyz)[(b * nfp + q) * 3 + d] = g[11 + d] + (e[0] * g[d] + e[1] * g[3 + d] + e[2] * g[6 + d]);
+    }
+  }
 }
 
 void DgGroup::TimeStepBits(unsigned long long *dt_min_bits, hipStream_t stream) {

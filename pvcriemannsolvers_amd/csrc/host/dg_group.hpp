@@ -38,6 +38,7 @@ struct DgGroup {
   std::vector<int32_t> h_face_cells;  // [face][3]: holder, sharer or -(1 + tag), faces and orientation
   DeviceArray<int32_t> face_cells;
   DeviceArray<double> fhat;           // [face][point][5]
+  DeviceArray<uint8_t> exact_star;    // exact solver: [face][point] sign of the last star velocity
   DeviceArray<DgTables> tables_d;
   // WENO limiter (MCFD_LIMITER_*), applied to every state that is set
   int limiter = 0;
@@ -69,6 +70,10 @@ struct DgGroup {
   // SolutionChanged.
   void Approximate(const double *values, hipStream_t stream);
   void Faces(int64_t *holder, int64_t *sharer, int32_t *holder_face, int32_t *sharer_face) const;
+  // Tag and quadrature points of every boundary face ([boundary face][point][3]).
+  void BoundaryPoints(std::vector<int32_t> *tags, std::vector<double> *xyz) const;
+  // Given states per boundary face point (BoundaryTable), or null: the tags' constants.
+  const double *bc_point_state = nullptr;
 };
 
 }  // namespace mcfd

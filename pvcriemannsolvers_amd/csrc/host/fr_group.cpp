@@ -49,7 +49,7 @@ void FrHexGroup::Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &schem
   for (int64_t b = 0; b < mesh.n_bfaces; ++b)
     if (mesh.bface_tags[b] >= n_bcs) throw std::invalid_argument("boundary tag without a BC entry");
   for (auto k : bc_kinds)
-    if (k < 0 || k > 3) throw std::invalid_argument("unknown BC kind");
+    if (k < 0 || k > 5) throw std::invalid_argument("unknown BC kind");
   h_bc_kind = bc_kinds;
   h_bc_state = bc_states;
   if (h_bc_kind.empty()) {  // keep the device arrays non-empty
@@ -136,6 +136,35 @@ void FrHexGroup::Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &schem
   }
   for (auto v : h_nbr)
     if (v == INT32_MIN) throw std::invalid_argument("cell face without a face record");
+  h_bface_of.assign(6 * nc, -1);
+  for (size_t f = conn.n_interior; f < conn.faces.size(); ++f)
+    h_bface_of[conn.faces[f].holder * 6 + conn.faces[f].holder_face] = (int32_t)(f - conn.n_interior);
+}
+
+void FrHexGroup::BoundaryPoints(std::vector<int32_t> *tags, std::vector<double> *xyz) const {
+  const int64_t nb = (int64_t)conn.faces.size() - conn.n_interior;
+  tags->assign(nb, 0);
+  xyz->assign(nb * q * q * 3, 0.0);
+  for (int64_t b = 0; b < nb; ++b) {
+    const auto &face = conn.faces[conn.n_interior + b];
+    (*tags)[b] = face.tag;
+    const int lf = face.holder_face, dir = kHexFaceNormalDir[lf];
+    const double end = kHexFaceSide[lf] ? 1.0 : -1.0;
+    const auto &j = conn.jac[face.holder];
+    const auto &x0 = corner0[face.holder];
+    for (int ia = 0; ia < q; ++ia)
+      for (int ib = 0; ib < q; ++ib) {
+        // reference point of flux point (a, b): the line end of fr/general.hpp:103-205
+        double s[3];
+        s[dir] = end;
+        if (dir == 2) { s[0] = x1d[ia]; s[1] = x1d[ib]; }
+        else if (dir == 1) { s[0] = x1d[ia]; s[2] = x1d[ib]; }
+        else { s[1] = x1d[ia]; s[2] = x1d[ib]; }
+        double *p = xyz->data() + ((b * q + ia) * q + ib) * 3;
+        for (int d = 0; d < 3; ++d)
+          p[d] = x0[d] + (s[0] + 1) * j[d] + (s[1] + 1) * j[3 + d] + (s[2] + 1) * j[6 + d];
+      }
+  }
 }
 
 void FrHexGroup::Upload(hipStream_t stream) {
@@ -151,6 +180,12 @@ void FrHexGroup::Upload(hipStream_t stream) {
   bc_state.Upload(h_bc_state, stream);
   if (!h_cell_case.empty()) cell_case.Upload(h_cell_case, stream);
   if (!h_tile_order.empty()) tile_order.Upload(h_tile_order, stream);
+  bface_of.Upload(h_bface_of, stream);
+  if (riemann == MCFD_RIEMANN_EXACT) {
+    const size_t n_star = size_t(n_cells) * 6 * q * q;
+    exact_star.Allocate(n_star);
+    Check(hipMemsetAsync(exact_star.ptr, 0, n_star, stream), "memset");
+  }
 }
 
 void FrHexGroup::BuildTileOrder(int tile_cells) {
@@ -204,6 +239,9 @@ FrHexArgs FrHexGroup::Args(int stage, double dt) const {
   a.xcd_remap = 1;
   a.tile_order = tile_order.ptr;
   a.dt_min_bits = nullptr;
+  a.exact_star = exact_star.ptr;
+  a.bc_point_state = nullptr;  // set by the caller when boundary functions are in use
+  a.bface_of = bface_of.ptr;
   a.diag_metric = diag_metric ? 1 : 0;
   a.mixed = mixed ? 1 : 0;
   switch (stage) {

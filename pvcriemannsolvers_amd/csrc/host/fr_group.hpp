@@ -28,6 +28,13 @@ struct FrHexGroup {
   DeviceArray<double> u0, u1, u2, res;
   DeviceArray<int32_t> nbr, finfo, geo_id, bc_kind, cell_case, tile_order;
   DeviceArray<double> geo, frames, bc_state, case_dt;
+  // exact solver: per (cell, local face, flux point) state of its solver
+  // object, the sign of the last star velocity (zero at creation)
+  DeviceArray<uint8_t> exact_star;
+  // boundary face index of each boundary cell face, [cell][6] (-1 elsewhere),
+  // for the per-point boundary values of BoundaryTable
+  std::vector<int32_t> h_bface_of;
+  DeviceArray<int32_t> bface_of;
 
   // Host-side construction; bc_kind/bc_state index the mesh's boundary tags.
   void Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &scheme,
@@ -44,6 +51,9 @@ struct FrHexGroup {
   // Kernel arguments of one pass: stage 0 residual, 1..3 SSP-RK3 stages.
   FrHexArgs Args(int stage, double dt) const;
   void SamplePoints(double *xyz) const;
+  // Tag and flux points of every boundary face ([boundary face][a * Q + b][3],
+  // (a, b) the in-face indices of the holder's local face).
+  void BoundaryPoints(std::vector<int32_t> *tags, std::vector<double> *xyz) const;
   void Faces(int64_t *holder, int64_t *sharer, int32_t *holder_face, int32_t *sharer_face,
              int32_t *fp_ijk) const;
 };

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../../include/minicfd.h"
+#include "bc_table.hpp"
 #include "common.hpp"
 #include "dg_group.hpp"
 #include "fr_group.hpp"
@@ -33,6 +34,7 @@ struct mcfd_solver {
   mcfd::FrHexGroup group;
   mcfd::DgGroup modal;
   mcfd::DeviceArray<unsigned long long> scratch;  // time-step reduction
+  mcfd::BoundaryTable bct;                         // boundary values that vary in space and time
   // The time step (fem.hpp:412-435) of the state an RK step ends with can be
   // folded into its stage-3 pass.  That is done while the caller reads a time
   // step between steps (as the reference's driver does, sourceless.cpp:190-
@@ -41,7 +43,12 @@ struct mcfd_solver {
   bool dt_pending = false;  // ... folded in by the last stage 3 and not read yet
   bool fuse_dt_wanted = false;
 
+  // Stage 0 forms the residual at time t; stages 1..3 advance the step that
+  // starts at t, whose residuals are formed at t, t + dt and t + dt / 2
+  // (temporal/rk.hpp:31-49 with ode.hpp:121-128 setting the time).
   void Stage(int stage, double dt) {
+    const double ts = stage == 2 ? t + dt : (stage == 3 ? t + dt / 2 : t);
+    const double *bc_points = bct.At(ts, stream);
     if (stage == 1 && dt_pending) fuse_dt_wanted = false;
     // the fused reduction needs the final state of the pass: not with a
     // limiter, which rewrites troubled cells afterwards
@@ -49,10 +56,12 @@ struct mcfd_solver {
     if (stage > 0) dt_fresh = dt_pending = false;
     if (fuse_dt) Check(mcfd::LaunchResetTimeStepBits(scratch.ptr, stream), "time-step reset");
     if (dg) {
+      modal.bc_point_state = bc_points;
       modal.Stage(stage, dt, stream, fuse_dt ? scratch.ptr : nullptr);
     } else {
       mcfd::FrHexArgs a = group.Args(stage, dt);
       a.dt_min_bits = fuse_dt ? scratch.ptr : nullptr;
+      a.bc_point_state = bc_points;
       Check(mcfd::LaunchFrHexStage(group.q, group.riemann, a, group.lt, stream), "FR stage launch");
     }
     if (fuse_dt) dt_fresh = dt_pending = true;
@@ -95,6 +104,19 @@ int mcfd_create(const mcfd_mesh_desc *mesh, const mcfd_scheme_desc *scheme, cons
       h->modal.Build(*mesh, *scheme, kinds, states);
     } else {
       h->group.Build(*mesh, *scheme, kinds, states);
+    }
+    {
+      std::vector<int32_t> tags;
+      std::vector<double> xyz;
+      int nfp;
+      if (h->dg) {
+        h->modal.BoundaryPoints(&tags, &xyz);
+        nfp = h->modal.nfp;
+      } else {
+        h->group.BoundaryPoints(&tags, &xyz);
+        nfp = h->group.q * h->group.q;
+      }
+      h->bct.Init(nfp, std::move(tags), std::move(xyz), states);
     }
     h->host_only = (scheme->flags & MCFD_FLAG_HOST_ONLY) != 0;
     if (h->host_only) return;
@@ -255,6 +277,21 @@ int mcfd_faces(const mcfd_solver *h, int64_t *holder, int64_t *sharer, int32_t *
   }
   h->group.Faces(holder, sharer, holder_face, sharer_face, fp_ijk);
   return MCFD_OK;
+}
+
+int mcfd_boundary_points(const mcfd_solver *h, int32_t tag, int64_t *n_points, double *xyz) {
+  if (!h || !n_points) return Fail(MCFD_EINVAL, "null argument");
+  if (tag < 0 || tag >= (int32_t)h->bct.fns.size()) return Fail(MCFD_EINVAL, "boundary tag out of range");
+  std::vector<double> p = h->bct.Points(tag);
+  *n_points = (int64_t)(p.size() / 3);
+  if (xyz) std::copy(p.begin(), p.end(), xyz);
+  return MCFD_OK;
+}
+
+int mcfd_set_boundary_function(mcfd_solver *h, int32_t tag, mcfd_bc_function f, void *user) {
+  MCFD_REQUIRE_DEVICE(h);
+  if (tag < 0 || tag >= (int32_t)h->bct.fns.size()) return Fail(MCFD_EINVAL, "boundary tag out of range");
+  return Guard([&] { h->bct.SetFunction(tag, f, user); });
 }
 
 int mcfd_limiter_flags(mcfd_solver *h, uint8_t *flags, int64_t *n_troubled) {

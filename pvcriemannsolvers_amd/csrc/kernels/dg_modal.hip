@@ -396,27 +396,37 @@ __global__ __launch_bounds__(256) void DgFaceFluxKernel(DgModalArgs a) {
   Offset(a.geo + (int64_t)holder * GS, tb->face_eta[hf][0][q], y);
   const State own = EvalTaylor<P>(a.ct + (int64_t)holder * 5 * N, y);
   State flux;
+  // the exact solver's state of this face point (ExactFlux), kept across calls
+  bool star = kRiemann == kExact ? a.exact_star[w] != 0 : false;
   if (sharer >= 0) {
     double ys[3];
     Offset(a.geo + (int64_t)sharer * GS, tb->face_eta[sf][orient][q], ys);
     const State other = EvalTaylor<P>(a.ct + (int64_t)sharer * 5 * N, ys);
-    flux = UpwindFlux<kRiemann>(frame, own, other);
+    flux = UpwindFlux<kRiemann>(frame, own, other, star);
   } else {
     const int tag = -sharer - 1;
     const int kind = a.bc_kind[tag];
+    // the tag's constant state, or this point's value of the tag's boundary
+    // function at the stage time
+    const double *gs = a.bc_point_state ? a.bc_point_state + (w - a.n_interior * NFP) * 5 : a.bc_state + tag * 5;
     State given;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) given.v[k] = a.bc_state[tag * 5 + k];
-    if (kind == 1) {
+    for (int k = 0; k < 5; ++k) given.v[k] = gs[k];
+    if (kind == kBcInviscidWall) {
       flux = WallFlux(frame, own);
-    } else if (kind == 2) {
+    } else if (kind == kBcSupersonicOutlet) {
       flux = StateFlux(frame, own);
-    } else if (kind == 3) {
+    } else if (kind == kBcSupersonicInlet) {
       flux = StateFlux(frame, given);
+    } else if (kind == kBcSubsonicInlet) {
+      flux = SubsonicInletFlux(frame, own, given);
+    } else if (kind == kBcSubsonicOutlet) {
+      flux = SubsonicOutletFlux(frame, own, given);
     } else {
-      flux = UpwindFlux<kRiemann>(frame, own, given);
+      flux = UpwindFlux<kRiemann>(frame, own, given, star);
     }
   }
+  if constexpr (kRiemann == kExact) a.exact_star[w] = star ? 1 : 0;
 #pragma unroll
   for (int k = 0; k < 5; ++k) a.fhat[w * 5 + k] = flux.v[k] * wq;
 }

@@ -53,8 +53,12 @@ struct DgModalArgs {
   const double *geo;      // [cell][kDgGeoStride]
   const int32_t *finfo;   // [cell][6]: face id << 8 | holder << 7 | orient << 3 | neighbour's local face
   const double *faces;    // [face][kDgFaceStride]
-  const int32_t *bc_kind; // [tag]: 0 smart, 1 inviscid wall, 2 supersonic outlet, 3 inlet
-  const double *bc_state; // [tag][5]
+  const int32_t *bc_kind; // [tag]: BcKind (euler.hpp)
+  const double *bc_state; // [tag][5] given state (conservative, or the subsonic parameters)
+  // If set, the given state of every boundary face point instead,
+  // [boundary face][point][5] (the tag's boundary function at the stage time)
+  const double *bc_point_state;
+  int64_t n_interior;     // interior faces, which come first
   const DgTables *tables;
   int64_t n_cells;
   double dt;
@@ -70,6 +74,9 @@ struct DgModalArgs {
   double *fhat;
   const int32_t *face_cells;
   int64_t n_faces;
+  // exact solver only: [face][point] sign of each face point's last star
+  // velocity (ExactFlux), zero-initialised at creation, updated every pass
+  uint8_t *exact_star;
 };
 
 // Ct = C S for every cell of `us` into `ct`.

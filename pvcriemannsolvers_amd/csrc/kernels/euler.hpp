@@ -337,11 +337,14 @@ MCFD_HD State NearVacuum(State const &l, State const &r) {  // exact.hpp:293-320
   for (int i = 0; i < 5; ++i) z.v[i] = 0;
   return z;
 }
-// exact.hpp:36-43,62-138.  The reference keeps `star_u` in the solver object
-// across calls and reuses a stale value for the passive scalars in vacuum
-// branches; here it starts at 0 on every call (the value a fresh solver holds).
-MCFD_HD State ExactFlux(State const &l, State const &r) {
-  double star_u = 0.0;
+// exact.hpp:36-43,62-138.  Every face point has its own solver object in
+// the reference (fem.hpp:95-115), and that object keeps `star_u`, the star
+// velocity of its last ordinary solve (exact.hpp:29,116), across calls.  The
+// passive tangential velocities take the side given by its sign
+// (exact.hpp:49-57), also in the vacuum branches, which do not update it.
+// `star_positive` carries that sign in and out; a fresh solver holds 0.
+MCFD_HD State ExactFlux(State const &l, State const &r, bool &star_positive) {
+  bool sp = star_positive;
   State res;
   for (int i = 0; i < 5; ++i) res.v[i] = 0;
   bool done = false;
@@ -385,7 +388,7 @@ MCFD_HD State ExactFlux(State const &l, State const &r) {
       State star;
       star.v[0] = 0; star.v[2] = 0; star.v[3] = 0; star.v[4] = x;
       star.v[1] = 0.5 * (r.v[1] + wr.F(x) + l.v[1] - wl.F(x));
-      star_u = star.v[1];
+      sp = star.v[1] > 0;
       if (0 < star.v[1]) {
         res = (x >= l.v[4]) ? NearShock(l, star, true) : NearFan(l, star, true);
       } else {
@@ -395,22 +398,25 @@ MCFD_HD State ExactFlux(State const &l, State const &r) {
       res = NearVacuum(l, r);
     }
   }
-  res.v[2] = star_u > 0 ? l.v[2] : r.v[2];
-  res.v[3] = star_u > 0 ? l.v[3] : r.v[3];
+  res.v[2] = sp ? l.v[2] : r.v[2];
+  res.v[3] = sp ? l.v[3] : r.v[3];
+  star_positive = sp;
   return NormalFlux(res);
 }
 
 enum RiemannKind { kExact = 0, kHllc = 1, kRoe = 2 };
 
 // rotated/euler.hpp:153-159: global conservative pair -> global flux.
+// `star_positive` is the exact solver's per-point state (see ExactFlux); the
+// other solvers keep none.
 template <int kRiemann>
-MCFD_HD State UpwindFlux(FaceFrame const &f, State const &cl, State const &cr) {
+MCFD_HD State UpwindFlux(FaceFrame const &f, State const &cl, State const &cr, bool &star_positive) {
   State l = ToPrimitive(cl), r = ToPrimitive(cr);
   GlobalToNormal(f, &l);
   GlobalToNormal(f, &r);
   State flux;
   if constexpr (kRiemann == kExact) {
-    flux = ExactFlux(l, r);
+    flux = ExactFlux(l, r, star_positive);
   } else if constexpr (kRiemann == kHllc) {
     flux = HllcFlux(l, r);
   } else {
@@ -418,6 +424,24 @@ MCFD_HD State UpwindFlux(FaceFrame const &f, State const &cl, State const &cr) {
   }
   NormalToGlobal(f, &flux);
   return flux;
+}
+
+// The upwind flux as the FR stage kernels evaluate it: Roe on primitives
+// recovered with one reciprocal of rho per side (FastPrimitive, FastRoeFlux),
+// the other solvers as UpwindFlux.
+template <int kRiemann>
+__device__ __forceinline__ State FaceFlux(FaceFrame const &f, State const &cl, State const &cr, bool &star) {
+  if constexpr (kRiemann == kRoe) {
+    double inv_l, inv_r;
+    State l = FastPrimitive(cl, &inv_l), r = FastPrimitive(cr, &inv_r);
+    GlobalToNormal(f, &l);
+    GlobalToNormal(f, &r);
+    State flux = FastRoeFlux(l, r, inv_l, inv_r);
+    NormalToGlobal(f, &flux);
+    return flux;
+  } else {
+    return UpwindFlux<kRiemann>(f, cl, cr, star);
+  }
 }
 
 // rotated/euler.hpp:160-174 (default branch): (0, p n, 0).
@@ -437,6 +461,61 @@ MCFD_HD State StateFlux(FaceFrame const &f, State const &c) {
   NormalToGlobal(f, &flux);
   return flux;
 }
+
+constexpr double kGasR = 287.05;  // riemann/euler/types.hpp:206-208
+
+// rotated/euler.hpp:181-246 (Carlson 2011, 2.7 subsonic inflow), default
+// branch: the boundary state from the interior's outgoing Riemann invariant
+// and the given total conditions, given = (p_total, u_cos, v_cos, w_cos,
+// T_total) with the flow direction cosines in global coordinates.
+MCFD_HD State SubsonicInletFlux(FaceFrame const &f, State const &c, State const &given) {
+  State p = ToPrimitive(c);
+  const double c_in = SoundSpeed(p.v[0], p.v[4]);
+  GlobalToNormal(f, &p);
+  const double riemann_in = -p.v[1] - c_in * (2 / kGm1);
+  const double u_cos = given.v[1], v_cos = given.v[2], w_cos = given.v[3], t_total = given.v[4];
+  const double a = f.n[0] * u_cos + f.n[1] * v_cos + f.n[2] * w_cos;
+  const double a2 = a * a;
+  const double gamma_r = kGamma * kGasR;
+  // larger root of qa x^2 + 2 qb x + qc = 0
+  const double qa = a2 + 2 / kGm1, qb = riemann_in;
+  const double qc = (kGm1 / 2) * (riemann_in * riemann_in) - a2 * gamma_r * t_total;
+  const double sqrt_delta = sqrt(qb * qb - qa * qc);
+  const double x1 = (-qb + sqrt_delta) / qa, x2 = (-qb - sqrt_delta) / qa;
+  const double c_b = x1 < x2 ? x2 : x1;
+  const double t_b = c_b * c_b / gamma_r;
+  const double mach = sqrt((t_total / t_b - 1) / (kGm1 / 2));
+  const double p_b = given.v[0] / pow(1 + (kGm1 / 2) * mach * mach, kGamma / kGm1);
+  const double speed = mach * c_b;
+  State b;
+  b.v[4] = p_b;
+  b.v[0] = p_b / kGasR / t_b;
+  b.v[1] = u_cos * speed;
+  b.v[2] = v_cos * speed;
+  b.v[3] = w_cos * speed;
+  GlobalToNormal(f, &b);
+  State flux = NormalFlux(b);
+  NormalToGlobal(f, &flux);
+  return flux;
+}
+
+// rotated/euler.hpp:263-280 (Carlson 2011, 2.4 pressure outflow), default
+// branch: the interior state at the given back pressure given[4], with the
+// interior's p / rho kept.
+MCFD_HD State SubsonicOutletFlux(FaceFrame const &f, State const &c, State const &given) {
+  State p = ToPrimitive(c);
+  const double rt = p.v[4] / p.v[0];
+  p.v[4] = given.v[4];
+  p.v[0] = p.v[4] / rt;
+  GlobalToNormal(f, &p);
+  State flux = NormalFlux(p);
+  NormalToGlobal(f, &flux);
+  return flux;
+}
+
+// Boundary kinds of the BC table (spatial/fem.hpp:196-222).
+enum BcKind { kBcSmart = 0, kBcInviscidWall = 1, kBcSupersonicOutlet = 2, kBcSupersonicInlet = 3,
+              kBcSubsonicInlet = 4, kBcSubsonicOutlet = 5 };
 
 }  // namespace mcfd
 

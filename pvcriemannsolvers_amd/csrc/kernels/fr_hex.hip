@@ -107,21 +107,6 @@ __device__ __forceinline__ int64_t LogicalBlock(int32_t remap) {
   return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
 }
 
-template <int kRiemann>
-__device__ __forceinline__ State FaceFlux(FaceFrame const &f, State const &cl, State const &cr) {
-  if constexpr (kRiemann == kRoe) {
-    double inv_l, inv_r;
-    State l = FastPrimitive(cl, &inv_l), r = FastPrimitive(cr, &inv_r);
-    GlobalToNormal(f, &l);
-    GlobalToNormal(f, &r);
-    State flux = FastRoeFlux(l, r, inv_l, inv_r);
-    NormalToGlobal(f, &flux);
-    return flux;
-  } else {
-    return UpwindFlux<kRiemann>(f, cl, cr);
-  }
-}
-
 // Column d of the local flux F~ = F J* at one solution point.
 template <bool kDiag>
 __device__ __forceinline__ void LocalFluxColumn(State const &c, State const &p, const double *js, int d,
@@ -399,6 +384,11 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
     const double scale = s_scale[t_cell[t] * 6 + f];
     State fu;
     double sgn = 1.0;
+    // the exact solver's state of this flux point, one copy per side: both
+    // sides solve the same problem with the same inputs every pass, so the
+    // two copies stay equal
+    const int64_t star_slot = ((c0 + t_cell[t]) * 6 + f) * NFP + ia * Q + ib;
+    bool star = kRiemann == kExact ? a.exact_star[star_slot] != 0 : false;
     if (nb >= 0) {
       const bool holder = (info >> 6) & 1;
       const double *nw = lt.lend[FaceLineIndex(Q, info & 7).side];
@@ -419,24 +409,34 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
         for (int m = 0; m < Q; ++m) vals[m] = raw[m][k];
         other.v[k] = Extrapolate<Q>(nw, vals);
       }
-      fu = FaceFlux<kRiemann>(frame, holder ? own : other, holder ? other : own);
+      fu = FaceFlux<kRiemann>(frame, holder ? own : other, holder ? other : own, star);
       sgn = holder ? 1.0 : -1.0;
     } else {
       const int tag = -nb - 1;
       const int kind = a.bc_kind[tag];
+      // the tag's constant state, or this point's value of the tag's
+      // boundary function at the stage time
+      const double *gs = a.bc_point_state
+          ? a.bc_point_state + ((int64_t)a.bface_of[(c0 + t_cell[t]) * 6 + f] * NFP + ia * Q + ib) * 5
+          : a.bc_state + tag * 5;
       State given;
 #pragma unroll
-      for (int k = 0; k < 5; ++k) given.v[k] = a.bc_state[tag * 5 + k];
-      if (kind == 1) {
+      for (int k = 0; k < 5; ++k) given.v[k] = gs[k];
+      if (kind == kBcInviscidWall) {
         fu = WallFlux(frame, own);
-      } else if (kind == 2) {
+      } else if (kind == kBcSupersonicOutlet) {
         fu = StateFlux(frame, own);
-      } else if (kind == 3) {
+      } else if (kind == kBcSupersonicInlet) {
         fu = StateFlux(frame, given);
+      } else if (kind == kBcSubsonicInlet) {
+        fu = SubsonicInletFlux(frame, own, given);
+      } else if (kind == kBcSubsonicOutlet) {
+        fu = SubsonicOutletFlux(frame, own, given);
       } else {
-        fu = FaceFlux<kRiemann>(frame, own, given);
+        fu = FaceFlux<kRiemann>(frame, own, given, star);
       }
     }
+    if constexpr (kRiemann == kExact) a.exact_star[star_slot] = star ? 1 : 0;
 #pragma unroll
     for (int k = 0; k < 5; ++k) gap[t][k] = fu.v[k] * (sgn * scale) - fn[t][k];
   }

@@ -38,8 +38,13 @@ struct FrHexArgs {
   const int32_t *geo_id; // [cell]
   const double *geo;     // [geo][kGeoStride]
   const double *frames;  // [frame][kFrameStride]
-  const int32_t *bc_kind;   // [tag]: 0 smart, 1 inviscid wall, 2 supersonic outlet, 3 inlet
-  const double *bc_state;   // [tag][5] conservative state
+  const int32_t *bc_kind;   // [tag]: BcKind (euler.hpp)
+  const double *bc_state;   // [tag][5] given state (conservative, or the subsonic parameters)
+  // If set, the given state of every boundary flux point instead:
+  // [boundary face][a * Q + b][5], with bface_of [cell][6] the boundary face
+  // index of each boundary cell face (the tag's boundary function at the stage time)
+  const double *bc_point_state;
+  const int32_t *bface_of;
   const int32_t *cell_case; // [cell] case index, or null (one time step for all cells)
   const double *case_dt;    // [case] time step when cell_case is set
   int64_t n_cells;
@@ -52,6 +57,9 @@ struct FrHexArgs {
   // If set, the time-step reduction of LaunchFrHexTimeStep over the state this
   // pass writes is folded into *dt_min_bits (same arithmetic, no extra pass).
   unsigned long long *dt_min_bits;
+  // exact solver only: [cell][local face][point] sign of each flux point's
+  // last star velocity (ExactFlux), zero-initialised at creation
+  uint8_t *exact_star;
 };
 
 // Launches one residual / RK-stage pass for Q points per direction and the

@@ -41,7 +41,8 @@ class FiniteElement(System):
 
     _RIEMANN = {"exact": C.RIEMANN_EXACT, "hllc": C.RIEMANN_HLLC, "roe": C.RIEMANN_ROE}
     _BC = {"smart": C.BC_SMART, "inviscid_wall": C.BC_INVISCID_WALL,
-           "supersonic_outlet": C.BC_SUPERSONIC_OUTLET, "supersonic_inlet": C.BC_SUPERSONIC_INLET}
+           "supersonic_outlet": C.BC_SUPERSONIC_OUTLET, "supersonic_inlet": C.BC_SUPERSONIC_INLET,
+           "subsonic_inlet": C.BC_SUBSONIC_INLET, "subsonic_outlet": C.BC_SUBSONIC_OUTLET}
 
     _LIMITER = {"none": C.LIMITER_NONE, "weno": C.LIMITER_WENO_FLAGGED, "weno_all": C.LIMITER_WENO_ALL,
                 "weno_eigen": C.LIMITER_WENO_EIGEN, "weno_eigen_all": C.LIMITER_WENO_EIGEN_ALL}
@@ -113,6 +114,33 @@ class FiniteElement(System):
         out = np.empty(self.column_size)
         C.check(C.load().mcfd_get_residual_column(self._h, C.ptr(out, C._dp)))
         return out
+
+    def boundary_points(self, tag):
+        """Points of the faces with boundary tag ``tag``, in the order a boundary function receives them."""
+        n = ctypes.c_int64()
+        C.check(C.load().mcfd_boundary_points(self._h, int(tag), ctypes.byref(n), None))
+        out = np.empty((n.value, 3))
+        C.check(C.load().mcfd_boundary_points(self._h, int(tag), ctypes.byref(n), C.ptr(out, C._dp)))
+        return out
+
+    def set_boundary_function(self, tag, func):
+        """Boundary values f(x, t) of a tag (spatial/fem.hpp:196-222): ``func(xyz[n, 3], t)``
+        returns the given states [n, 5] at the tag's points; None restores the constant state."""
+        if not hasattr(self, "_bc_callbacks"):
+            self._bc_callbacks = {}
+        if func is None:
+            C.check(C.load().mcfd_set_boundary_function(self._h, int(tag), C.BC_FUNCTION(), None))
+            self._bc_callbacks.pop(tag, None)
+            return
+
+        def trampoline(n, xyz, t, states, _user):
+            x = np.ctypeslib.as_array(xyz, shape=(n, 3))
+            out = np.ctypeslib.as_array(states, shape=(n, 5))
+            out[...] = func(x.copy(), t)
+
+        cb = C.BC_FUNCTION(trampoline)
+        self._bc_callbacks[tag] = cb  # kept alive while the library may call it
+        C.check(C.load().mcfd_set_boundary_function(self._h, int(tag), cb, None))
 
     def limiter_flags(self):
         """Troubled-cell flags of the last limiter pass (limiter/reconstruct.hpp:29-34)."""
@@ -259,6 +287,12 @@ class Batch:
         mode = ctypes.c_int32()
         C.check(C.load().mcfd_batch_autotune(self._h, int(repeats), C.ptr(tiles, C._i32p), ctypes.byref(mode)))
         return tiles, mode.value
+
+    def set_mode(self, mode, tiles=None):
+        """Fix the launch form: mode 1 = persistent launch per stage, 0 = one launch
+        per degree group; ``tiles`` = tile variant (0..2) per degree group, ascending degree."""
+        t = None if tiles is None else np.ascontiguousarray(tiles, np.int32)
+        C.check(C.load().mcfd_batch_set_mode(self._h, int(mode), C.ptr(t, C._i32p) if t is not None else None))
 
     def rk3_steps(self, n_steps):
         C.check(C.load().mcfd_batch_rk3_steps(self._h, int(n_steps)))

@@ -119,6 +119,21 @@ def riemann_flux(kind, frame, cl, cr):
     return out
 
 
+def face_fluxes(riemann, kind, frames, left, right=None, star_u=None):
+    """Fluxes at n points, each with its own solver; star_u (in/out) is each
+    point's exact-solver state (exact.hpp:29)."""
+    lib = load()
+    lib.oracle_face_fluxes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp]
+    frames, left = f64(frames).reshape(-1, 9), f64(left).reshape(-1, 5)
+    n = len(left)
+    right = f64(right).reshape(-1, 5) if right is not None else None
+    out = np.zeros((n, 5))
+    lib.oracle_face_fluxes(riemann, kind, n, _ptr(frames, _dp), _ptr(left, _dp),
+                           _ptr(right, _dp) if right is not None else None,
+                           _ptr(star_u, _dp) if star_u is not None else None, _ptr(out, _dp))
+    return out
+
+
 def wall_flux(frame, c):
     frame, c, out = f64(frame).reshape(9), f64(c), np.zeros(5)
     load().oracle_wall_flux(_ptr(frame, _dp), _ptr(c, _dp), _ptr(out, _dp))
@@ -135,8 +150,77 @@ def eigen_matrices(cv, frame):
     return l, r
 
 
-class FrOracle:
+class _BoundaryValues:
+    """Boundary values that vary in space and time (spatial/fem.hpp:196-222):
+    the caller's functions f(xyz, t) evaluated at the restatement's boundary
+    points, the other tags' constant states elsewhere."""
+
+    _prefix = ""
+
+    def boundary_points(self):
+        """(xyz [n, 3], tag [n]) of every boundary face point, in face order."""
+        lib = load()
+        fn = getattr(lib, f"oracle_{self._prefix}_boundary_points")
+        fn.restype = ctypes.c_int64
+        fn.argtypes = [ctypes.c_void_p, _dp, _i32p]
+        n = fn(self.h, None, None)
+        xyz, tags = np.zeros((n, 3)), np.zeros(n, np.int32)
+        fn(self.h, _ptr(xyz, _dp), _ptr(tags, _i32p))
+        return xyz, tags
+
+    def set_boundary_states(self, states):
+        """Given state at every boundary point ([n, 5]), or None for the tags' constants."""
+        lib = load()
+        fn = getattr(lib, f"oracle_{self._prefix}_set_boundary_states")
+        fn.argtypes = [ctypes.c_void_p, ctypes.c_int64, _dp]
+        if states is None:
+            fn(self.h, 0, None)
+            return
+        states = f64(states).reshape(-1, 5)
+        fn(self.h, len(states), _ptr(states, _dp))
+
+    def boundary_states_at(self, t, functions, constants):
+        """Given states at time t: functions {tag: f(xyz, t) -> [n, 5]}, constants [tag][5]."""
+        xyz, tags = self.boundary_points()
+        out = np.asarray(constants, dtype=np.float64)[tags].copy()
+        for tag, f in functions.items():
+            m = tags == tag
+            if m.any():
+                out[m] = f(xyz[m], t)
+        return out
+
+
+def rk3_timed(o, u, t, dt, functions, constants, limit=None):
+    """One SSP-RK3 step of temporal/rk.hpp:31-49 (operation order of
+    ode.hpp:121-128) with the boundary functions evaluated at the stage times
+    t, t + dt and t + dt / 2; ``limit`` is the SetSolutionColumn side effect."""
+    def next_solution(state, ts):
+        o.set_boundary_states(o.boundary_states_at(ts, functions, constants))
+        r = o.residual(state)
+        r *= dt
+        return state + r
+
+    def set_solution(v):
+        return limit(v) if limit is not None else v
+
+    u = f64(u)
+    u1 = set_solution(next_solution(u, t))
+    u_next = u.copy()
+    u_next *= 3
+    u_next += next_solution(u1, t + dt)
+    u_next /= 4
+    u2 = set_solution(u_next)
+    v = next_solution(u2, t + dt / 2)
+    v *= 2
+    v += u
+    v /= 3
+    return set_solution(v)
+
+
+class FrOracle(_BoundaryValues):
     """Nodal FR on hexahedra (spatial/fr/general.hpp restated)."""
+
+    _prefix = "fr"
 
     def __init__(self, mesh: dict, q: int, riemann: int, lobatto=False, c_next=1.0, bcs=()):
         lib = load()
@@ -222,8 +306,10 @@ class FrOracle:
         return load().oracle_fr_time_residuals(self.h, _ptr(u, _dp), _ptr(r, _dp), n)
 
 
-class DgOracle:
+class DgOracle(_BoundaryValues):
     """Modal DG on hexahedra or tetrahedra (spatial/dg/general.hpp restated)."""
+
+    _prefix = "dg"
 
     def __init__(self, mesh: dict, degree: int, riemann: int, bcs=()):
         lib = load()

@@ -1,0 +1,171 @@
+"""Boundary conditions on the device against the oracle, through the C ABI.
+
+* Subsonic inlet and outlet (riemann/rotated/euler.hpp:181-280, the families
+  of dg/general.hpp:146-183 and fr/general.hpp:566-616): FR and modal-DG
+  residuals, and the point-wise fluxes of both stage-kernel forms.
+* Boundary values that vary in space and time (the function argument of
+  SetSmartBoundary / SetSupersonicInlet, spatial/fem.hpp:196-222): the
+  library evaluates the caller's function at the stage times t, t + dt and
+  t + dt / 2 of each fused SSP-RK3 step; the oracle's step is driven with the
+  same function at the same times (oracle_lib.rk3_timed).  Residuals at a set
+  time follow mcfd_set_time (System::SetTime).
+
+Tolerances as the other parity tests: 1e-12 relative to the max norm.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from pvcriemannsolvers_amd import meshgen, spatial
+from pvcriemannsolvers_amd import riemann as rm
+
+pytestmark = pytest.mark.gpu
+
+GAMMA = 1.4
+KIND = {"smart": 0, "inviscid_wall": 1, "supersonic_outlet": 2, "supersonic_inlet": 3, "subsonic_inlet": 4,
+        "subsonic_outlet": 5}
+RIEMANN = {"exact": 0, "hllc": 1, "roe": 2}
+
+
+def cons(rho, vel, p):
+    rho = np.asarray(rho, dtype=np.float64)
+    vel = np.asarray(vel, dtype=np.float64)
+    u = np.zeros(rho.shape + (5,))
+    u[..., 0] = rho
+    u[..., 1:4] = rho[..., None] * vel
+    u[..., 4] = p / (GAMMA - 1) + 0.5 * rho * np.sum(vel * vel, axis=-1)
+    return u
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+# air-like state: rho ~ 1.2, p ~ 1e5, u ~ 60 along +x (subsonic), T = p / (rho R)
+def air_state(x, seed):
+    rng = np.random.default_rng(seed)
+    shape = x.shape[:-1]
+    rho = 1.2 * (1 + 0.02 * np.sin(2 * np.pi * x[..., 0]) + 0.01 * rng.random(shape))
+    vel = np.stack([60 + 3 * np.cos(2 * np.pi * x[..., 1]), 2 * rng.random(shape) - 1, np.zeros(shape)], axis=-1)
+    p = 1e5 * (1 + 0.01 * np.cos(2 * np.pi * x[..., 0]))
+    return cons(rho, vel, p)
+
+
+SUBSONIC_BCS = [("subsonic_inlet", [1.03e5, 1.0, 0.0, 0.0, 300.0]), ("subsonic_outlet", [0, 0, 0, 0, 0.98e5]),
+                ("inviscid_wall", None), ("inviscid_wall", None), ("inviscid_wall", None), ("inviscid_wall", None)]
+
+
+def oracle_bcs(bcs):
+    return [(KIND[k], s if s is not None else [0.0] * 5) for k, s in bcs]
+
+
+@pytest.mark.parametrize("riemann", ["hllc", "roe"])
+@pytest.mark.parametrize("degree", [1, 3])
+def test_fr_subsonic_boundaries(riemann, degree):
+    mesh = meshgen.box_hex(5, 3, 2, hi=(1.0, 0.6, 0.4), shuffle_seed=4)
+    o = ol.FrOracle(mesh, q=degree + 1, riemann=RIEMANN[riemann], bcs=oracle_bcs(SUBSONIC_BCS))
+    g = spatial.FiniteElement(mesh, degree=degree, riemann=riemann, bcs=SUBSONIC_BCS)
+    u = air_state(o.node_coords(), degree)
+    g.set_solution_column(u.reshape(-1))
+    assert rel(g.get_residual_column(), o.residual(u).reshape(-1)) < 1e-12
+    dt = o.time_step(u.reshape(-1), 0.1)
+    ref = o.rk3(u.reshape(-1), dt, n_steps=3)
+    for s in range(3):
+        g.rk3_update(s * dt, dt)
+    assert rel(g.get_solution_column(), ref) < 1e-12
+    g.close()
+
+
+@pytest.mark.parametrize("name", ["tets", "hexes"])
+def test_dg_subsonic_boundaries(name):
+    mesh = (meshgen.box_tet(4, 3, 2, hi=(1.0, 0.75, 0.5), jitter=0.1, seed=3) if name == "tets"
+            else meshgen.box_hex(5, 3, 2, hi=(1.0, 0.6, 0.4), shuffle_seed=6))
+    o = ol.DgOracle(mesh, degree=2, riemann=1, bcs=oracle_bcs(SUBSONIC_BCS))
+    g = spatial.FiniteElement(mesh, degree=2, scheme="dg", riemann="hllc", bcs=SUBSONIC_BCS)
+    vals = air_state(g.sample_points(), 7)
+    g.approximate(vals)
+    u = o.approximate(vals)
+    assert rel(g.get_solution_column(), u) < 1e-13
+    assert rel(g.get_residual_column(), o.residual(u)) < 1e-12
+    g.close()
+
+
+def test_subsonic_point_fluxes_match_oracle():
+    rng = np.random.default_rng(8)
+    n = 512
+    q, _ = np.linalg.qr(rng.standard_normal((n, 3, 3)))
+    frames = q * np.sign(np.linalg.det(q))[:, None, None]
+    normal = frames[:, 0]
+    d = -normal + 0.3 * rng.standard_normal((n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    d[(d * normal).sum(1) > -0.3] = -normal[(d * normal).sum(1) > -0.3]
+    left = cons(1.2 + 0.1 * rng.random(n), 50 * d + 5 * rng.standard_normal((n, 3)), 1e5 * (1 + 0.05 * rng.random(n)))
+    given_in = np.stack([1.03e5 * (1 + 0.01 * rng.random(n)), d[:, 0], d[:, 1], d[:, 2], 300 + rng.random(n)], 1)
+    given_out = np.zeros((n, 5))
+    given_out[:, 4] = 0.97e5
+    for form in ("fr", "dg"):
+        s = rm.Rotated("hllc", n, form)
+        got = s.get_flux_on_subsonic_inlet(frames, left, given_in)
+        want = ol.face_fluxes(1, 3, frames, left, given_in)
+        assert (np.abs(got - want).max(1) / np.abs(want).max(1)).max() < 1e-12
+        got = s.get_flux_on_subsonic_outlet(frames, left, given_out)
+        want = ol.face_fluxes(1, 4, frames, left, given_out)
+        assert (np.abs(got - want).max(1) / np.abs(want).max(1)).max() < 1e-13
+
+
+# ---------------------------------------------------------------- time-dependent boundary values
+LEFT, RIGHT = cons(1.0, (0.1, 0.0, 0.0), 1.0), cons(0.125, (0.0, 0.0, 0.0), 0.1)
+
+
+def pulsing_left(xyz, t):
+    """A smart-boundary state that varies over the face and in time."""
+    s = np.broadcast_to(LEFT, (len(xyz), 5)).copy()
+    s[:, 0] *= 1 + 0.05 * np.sin(40 * t + 10 * xyz[:, 1])
+    s[:, 4] *= 1 + 0.05 * np.cos(30 * t + 7 * xyz[:, 2])
+    return s
+
+
+@pytest.mark.parametrize("scheme", ["fr", "dg"])
+def test_time_dependent_smart_boundary(scheme):
+    bcs = [("smart", LEFT), ("smart", RIGHT)] + [("inviscid_wall", None)] * 4
+    constants = np.array([LEFT, RIGHT] + [[0.0] * 5] * 4)
+    funcs = {0: pulsing_left}
+    if scheme == "fr":
+        mesh = meshgen.box_hex(6, 2, 2, hi=(1.0, 0.2, 0.2), shuffle_seed=3)
+        o = ol.FrOracle(mesh, q=3, riemann=1, bcs=oracle_bcs(bcs))
+        g = spatial.FiniteElement(mesh, degree=2, riemann="hllc", bcs=bcs)
+        x = o.node_coords()
+        u = np.where((x[..., 0] < 0.5)[..., None], LEFT, RIGHT).reshape(-1)
+        g.set_solution_column(u)
+        limit = None
+    else:
+        mesh = meshgen.box_tet(6, 2, 1, hi=(1.0, 0.3, 0.15), jitter=0.1, seed=4)
+        o = ol.DgOracle(mesh, degree=2, riemann=1, bcs=oracle_bcs(bcs))
+        o.set_limiter(1)
+        g = spatial.FiniteElement(mesh, degree=2, scheme="dg", riemann="hllc", bcs=bcs, limiter="weno")
+        vals = np.where((g.sample_points()[..., 0] < 0.5)[..., None], LEFT, RIGHT)
+        g.approximate(vals)
+        u, _ = o.limit(o.approximate(vals))
+        limit = lambda v: o.limit(v)[0]  # noqa: E731
+    g.set_boundary_function(0, pulsing_left)
+    # the boundary points the function receives are on the x = 0 face
+    assert np.abs(g.boundary_points(0)[:, 0]).max() < 1e-14
+    # System path: the residual at a set time
+    g.set_time(0.013)
+    o.set_boundary_states(o.boundary_states_at(0.013, funcs, constants))
+    assert rel(g.get_residual_column(), o.residual(u)) < 1e-12
+    # fused RK3 steps: the function at t, t + dt, t + dt / 2
+    t, dt = 0.0, 2e-3
+    for _ in range(4):
+        g.rk3_update(t, dt)
+        u = ol.rk3_timed(o, u, t, dt, funcs, constants, limit)
+        t += dt
+    assert rel(g.get_solution_column(), u) < 1e-12
+    # removing the function restores the constant state
+    g.set_boundary_function(0, None)
+    o.set_boundary_states(None)
+    g.set_solution_column(u)
+    if limit is not None:
+        u = limit(u)
+    assert rel(g.get_residual_column(), o.residual(u)) < 1e-12
+    g.close()

@@ -151,7 +151,7 @@ def test_free_stream_and_conservation_64cubed():
     assert np.array_equal(r.reshape(-1), g.get_residual_column())
 
 
-@pytest.mark.parametrize("degree", [3, 4])
+@pytest.mark.parametrize("degree", [1, 2, 3, 4])
 def test_mixed_precision_form(degree):
     """Mixed form: FP32 flux-column contractions; FP64 states, traces, Roe, RK.
 
@@ -184,10 +184,10 @@ def _tube_case(n, degree, rho_l, u_l, p_l, rho_r, u_r, p_r):
     return mesh, degree, bcs, left, right
 
 
-def test_batched_cases_match_single_case_oracle():
-    """Several shock tubes of uneven size and degree in one batch (persistent
-    launch per stage, graph-captured step, per-case dt) == the oracle per case."""
-    rng = np.random.default_rng(12)
+def _tube_batch(precision, seed=12):
+    """Shock tubes of uneven size and degree p in {1, 2, 3} in one batch, with
+    their single-case oracles, initial columns and per-case time steps."""
+    rng = np.random.default_rng(seed)
     specs = [(7, 1), (5, 2), (9, 3), (4, 1), (6, 3), (3, 2)]
     cases, refs = [], []
     for n, p in specs:
@@ -197,7 +197,7 @@ def test_batched_cases_match_single_case_oracle():
         mesh, degree, bcs, left, right = _tube_case(n, p, rl, ul, pl, rr, ur, pr)
         cases.append((mesh, degree, bcs))
         refs.append((left, right))
-    batch = spatial.Batch(cases, riemann="hllc")
+    batch = spatial.Batch(cases, riemann="hllc", precision=precision)
     oracles, states, dts = [], [], []
     for i, (mesh, degree, bcs) in enumerate(cases):
         o = ol.FrOracle(mesh, q=degree + 1, riemann=1,
@@ -212,15 +212,55 @@ def test_batched_cases_match_single_case_oracle():
         oracles.append(o)
         states.append(u)
         dts.append(o.time_step(u, 0.1))
+    return batch, oracles, states, dts
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("mode,tile", [(0, 0), (1, 0), (1, 1), (1, 2)])
+def test_batched_forms_match_single_case_oracle(precision, mode, tile):
+    """Every launch form of the batch, forced: one launch per degree group
+    (mode 0) and the persistent launch (mode 1, FrHexBatchKernel) with each
+    tile variant, in FP64 and in the mixed form, graph-captured, per-case dt.
+
+    FP64: each case equals its single-case oracle to 1e-12 (relative, max
+    norm) after 3 RK3 steps.  Mixed (FP32 flux contractions): the density and
+    energy differ from the FP64 oracle by at most 1e-8 in the L2 norm of
+    mesh/part.hpp:1022-1036.
+    """
+    batch, oracles, states, dts = _tube_batch(precision)
     gdt = batch.get_time_steps(0.1)
     assert np.allclose(gdt, dts, rtol=1e-14, atol=0)
     batch.set_time_steps(np.array(dts))
-    batch.autotune(repeats=1)
+    batch.set_mode(mode, [tile] * 3)
+    batch.rk3_steps(3)
+    batch.synchronize()
+    for i, o in enumerate(oracles):
+        ref = o.rk3(states[i], dts[i], n_steps=3)
+        got = batch.get_solution_column(i)
+        if precision == "fp64":
+            assert rel(got, ref) < 1e-12, i
+        else:
+            w = o.node_weights()
+            d = (got - ref).reshape(w.shape + (5,))
+            for k in (0, 4):
+                l2 = np.sqrt((w * d[..., k] ** 2).sum())
+                assert l2 <= 1e-8, (i, k, l2)
+    batch.close()
+
+
+def test_batched_autotune_keeps_results():
+    """The autotuner's choice (whatever it is on this device) gives the same
+    states as the oracle; its trial launches leave no trace in the state."""
+    batch, oracles, states, dts = _tube_batch("fp64", seed=13)
+    batch.set_time_steps(np.array(dts))
+    tiles, mode = batch.autotune(repeats=1)
+    assert mode in (0, 1) and all(t % 16 in (0, 1, 2) for t in tiles[:3])
     batch.rk3_steps(3)
     batch.synchronize()
     for i, o in enumerate(oracles):
         ref = o.rk3(states[i], dts[i], n_steps=3)
         assert rel(batch.get_solution_column(i), ref) < 1e-12, i
+    batch.close()
 
 
 @pytest.mark.parametrize("degree", [1, 3])

@@ -1,0 +1,141 @@
+"""CPU pins of the boundary conditions added to the oracle.
+
+* Subsonic inlet (riemann/rotated/euler.hpp:181-246, Carlson 2011 2.7): an
+  interior state that already carries the given total pressure, total
+  temperature and flow direction, flowing in subsonically, is reproduced by
+  the boundary state, so the flux is the interior's physical flux.  A wrong
+  root of the quadratic, a sign of the Riemann invariant or an exponent of the
+  isentropic relations moves it off.
+* Subsonic outlet (:263-280, Carlson 2.4): the boundary state is the interior
+  at the back pressure with p / rho kept; closed form.
+* The FR residual with subsonic inlets and outlets equals the independent
+  nodal-DG weak form (oracle/nodal_dg.py) with the same boundary fluxes.
+* The Python-driven SSP-RK3 step with boundary values evaluated at the stage
+  times equals the restatement's own step bit for bit when the values do not
+  depend on time.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from pvcriemannsolvers_amd import meshgen
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import nodal_dg  # noqa: E402
+
+GAMMA, R = 1.4, 287.05
+
+
+def frames(n, seed):
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.standard_normal((n, 3, 3)))
+    return q * np.sign(np.linalg.det(q))[:, None, None]
+
+
+def cons(rho, vel, p):
+    vel = np.asarray(vel, dtype=np.float64)
+    return np.array([rho, *(rho * vel), p / (GAMMA - 1) + 0.5 * rho * vel @ vel])
+
+
+def physical_flux(u, n):
+    rho = u[0]
+    vel = u[1:4] / rho
+    p = (GAMMA - 1) * (u[4] - 0.5 * rho * vel @ vel)
+    un = vel @ n
+    return np.array([rho * un, *(u[1:4] * un + p * n), (u[4] + p) * un])
+
+
+def test_subsonic_inlet_reproduces_a_consistent_interior():
+    rng = np.random.default_rng(1)
+    fr = frames(64, 2)
+    left, given, want = [], [], []
+    for i in range(64):
+        n = fr[i, 0]
+        rho, p = rng.uniform(0.5, 2.0), rng.uniform(0.5, 2.0) * 1e5
+        c = np.sqrt(GAMMA * p / rho)
+        d = -n + 0.4 * rng.standard_normal(3)  # into the domain
+        d /= np.linalg.norm(d)
+        if d @ n > -0.3:  # the reference asserts inflow (euler.hpp:197)
+            d = -n
+        vel = rng.uniform(0.05, 0.8) * c * d   # subsonic
+        mach2 = vel @ vel / c ** 2
+        factor = 1 + (GAMMA - 1) / 2 * mach2
+        t = p / (rho * R)
+        given.append([p * factor ** (GAMMA / (GAMMA - 1)), *d, t * factor])
+        u = cons(rho, vel, p)
+        left.append(u)
+        want.append(physical_flux(u, n))
+    got = ol.face_fluxes(1, 3, fr, np.array(left), np.array(given))
+    want = np.array(want)
+    # low Mach numbers recover M from sqrt(T_total / T - 1): about 1e-13 of cancellation
+    assert (np.abs(got - want).max(axis=1) / np.abs(want).max(axis=1)).max() < 1e-11
+
+
+def test_subsonic_outlet_closed_form():
+    rng = np.random.default_rng(3)
+    fr = frames(64, 4)
+    left, given, want = [], [], []
+    for i in range(64):
+        n = fr[i, 0]
+        rho, p = rng.uniform(0.5, 2.0), rng.uniform(0.5, 2.0)
+        vel = 0.3 * n + 0.2 * rng.standard_normal(3)
+        pb = rng.uniform(0.5, 2.0)
+        u = cons(rho, vel, p)
+        left.append(u)
+        given.append([0, 0, 0, 0, pb])
+        want.append(physical_flux(cons(rho * pb / p, vel, pb), n))
+    got = ol.face_fluxes(1, 4, fr, np.array(left), np.array(given))
+    want = np.array(want)
+    assert (np.abs(got - want).max(axis=1) / np.abs(want).max(axis=1)).max() < 1e-13
+
+
+def test_fr_residual_with_subsonic_boundaries_equals_weak_form():
+    mesh = meshgen.box_hex(2, 2, 1, hi=(0.6, 0.4, 0.3), shuffle_seed=3)
+    inlet = [1.2e5, 1.0, 0.0, 0.0, 300.0]
+    bcs = [(4, inlet), (5, [0, 0, 0, 0, 0.9e5]), (1, [0] * 5), (1, [0] * 5), (2, [0] * 5), (1, [0] * 5)]
+    fr = ol.FrOracle(mesh, q=3, riemann=1, bcs=bcs)
+    weak = nodal_dg.NodalDgWeak(mesh, 3, bcs=bcs)
+    x = weak.x_nodes
+    rng = np.random.default_rng(5)
+    rho = 1.2 + 0.1 * rng.random(x.shape[:-1])
+    vel = np.stack([60 + 5 * rng.random(x.shape[:-1]), rng.random(x.shape[:-1]), rng.random(x.shape[:-1])], -1)
+    p = 1e5 * (1 + 0.05 * rng.random(x.shape[:-1]))
+    u = np.concatenate([rho[..., None], rho[..., None] * vel,
+                        (p / (GAMMA - 1) + 0.5 * rho * (vel ** 2).sum(-1))[..., None]], -1).reshape(-1)
+
+    def boundary(kind, frame, u_in, given):
+        return ol.face_fluxes(1, 3 if kind == 4 else 4, frame[None], u_in[None], np.asarray(given)[None])[0]
+
+    r_fr = fr.residual(u)
+    r_weak = weak.residual(u, lambda f, ul, ur: ol.riemann_flux(1, f, ul, ur), boundary)
+    assert np.abs(r_fr - r_weak).max() / np.abs(r_weak).max() < 1e-12
+
+
+@pytest.mark.parametrize("scheme", ["fr", "dg"])
+def test_timed_rk3_equals_restatement_step(scheme):
+    left, right = cons(1.0, (0.1, 0, 0), 1.0), cons(0.125, (0, 0, 0), 0.1)
+    if scheme == "fr":
+        mesh = meshgen.box_hex(4, 2, 2, hi=(1.0, 0.2, 0.2), shuffle_seed=1)
+        o = ol.FrOracle(mesh, q=3, riemann=1, bcs=[(0, left), (0, right)] + [(1, [0] * 5)] * 4)
+        x = o.node_coords()
+        u = np.where((x[..., 0] < 0.5)[..., None], left, right).reshape(-1) * (1 + 1e-3 * np.sin(7 * x[..., :1].repeat(5, -1).reshape(-1)))
+        limit = None
+    else:
+        mesh = meshgen.box_tet(4, 4, 1, hi=(1.0, 1.0, 0.25), jitter=0.1, seed=2)
+        o = ol.DgOracle(mesh, degree=2, riemann=1, bcs=[(0, left), (0, right)] + [(1, [0] * 5)] * 4)
+        o.set_limiter(1)
+        x = o.cells()["xyz"]
+        u, _ = o.limit(o.approximate(np.where((x[..., 0] < 0.5)[..., None], left, right)))
+        limit = lambda v: o.limit(v)[0]  # noqa: E731
+    constants = np.array([left, right] + [[0.0] * 5] * 4)
+    dt = 1e-3
+    want = o.rk3(u, dt, 1)
+    # the smart boundaries as functions that return their constants
+    funcs = {0: lambda xyz, t: np.broadcast_to(left, (len(xyz), 5)),
+             1: lambda xyz, t: np.broadcast_to(right, (len(xyz), 5))}
+    got = ol.rk3_timed(o, u, 0.0, dt, funcs, constants, limit)
+    o.set_boundary_states(None)
+    np.testing.assert_array_equal(got, want)
