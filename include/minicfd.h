@@ -239,6 +239,18 @@ int mcfd_limiter_flags(mcfd_solver *h, uint8_t *flags, int64_t *n_troubled);
 int mcfd_face_fluxes(int32_t riemann, int32_t form, int32_t kind, int64_t n, const double *frames,
                      const double *left, const double *right, uint8_t *exact_star, double *flux);
 
+/* mcfd_face_fluxes without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (serving
+ * loops and harnesses that forbid hipMalloc).  The points are staged through
+ * a fixed device buffer that the library holds on each GPU until the process
+ * ends (about 3 MiB), so a call takes at most MCFD_NOALLOC_MAX_POINTS points
+ * and returns MCFD_EINVAL beyond that; it runs on the current device's
+ * default stream and synchronizes it.  Calls that overlap on one GPU share
+ * the buffer and race.  Same arguments and results as mcfd_face_fluxes. */
+#define MCFD_NOALLOC_MAX_POINTS 16384
+int mcfd_face_fluxes_noalloc(int32_t riemann, int32_t form, int32_t kind, int64_t n, const double *frames,
+                             const double *left, const double *right, uint8_t *exact_star, double *flux);
+
 /* Stream synchronisation and per-kernel timing of n RK3 steps with HIP
  * events on the handle's stream: total ms, summed ms of the stage kernels,
  * and (if stage_ms is not null) the ms of each of the 3 * n_steps stage

@@ -25,6 +25,7 @@ BC_SMART, BC_INVISCID_WALL, BC_SUPERSONIC_OUTLET, BC_SUPERSONIC_INLET = 0, 1, 2,
 BC_SUBSONIC_INLET, BC_SUBSONIC_OUTLET = 4, 5
 FLUX_UPWIND, FLUX_INVISCID_WALL, FLUX_SUPERSONIC_OUTLET, FLUX_SUBSONIC_INLET, FLUX_SUBSONIC_OUTLET = 0, 1, 2, 3, 4
 FLUX_FORM_FR, FLUX_FORM_DG = 0, 1
+NOALLOC_MAX_POINTS = 16384
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _i64p = ctypes.POINTER(ctypes.c_int64)
@@ -74,6 +75,8 @@ SIGNATURES = {
     "mcfd_set_boundary_function": [ctypes.c_void_p, ctypes.c_int32, BC_FUNCTION, ctypes.c_void_p],
     "mcfd_face_fluxes": [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, _dp, _dp, _dp,
                          ctypes.POINTER(ctypes.c_uint8), _dp],
+    "mcfd_face_fluxes_noalloc": [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, _dp, _dp, _dp,
+                                 ctypes.POINTER(ctypes.c_uint8), _dp],
     "mcfd_synchronize": [ctypes.c_void_p],
     "mcfd_time_rk3": [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32, _dp, _dp, _dp],
     "mcfd_batch_create": [ctypes.c_int32, ctypes.POINTER(MeshDesc), _i32p, ctypes.POINTER(SchemeDesc),
