@@ -27,10 +27,18 @@
 #include <algorithm>
 
 #include "euler.hpp"
+#include "fr_common.hpp"
 
 namespace mcfd {
 
 namespace {
+
+using fr::Extrapolate;
+using fr::FaceLineIndex;
+using fr::LineIndex;
+using fr::LocalFluxColumn;
+using fr::LogicalBlock;
+using fr::SkewedLineIndex;
 
 template <int Q, int Cells = 256 / (Q * Q * Q), int Threads = 256, bool OneRound = false>
 struct Shape {
@@ -58,81 +66,6 @@ struct Shape {
   // states, F~ columns / flux gaps, mixed reference values, face area scales
   static constexpr int kLdsDoubles = kCells * 5 * kPad + kCells * kScratch + kCells * 15 + kCells * 6;
 };
-
-// Line of solution points through in-face point (a, b) of local face f:
-// node(m) = a * A + b * B + m * S.  Branch-free for lane-varying f.
-struct LineIndex { int a_mul, b_mul, stride, side, dir; };
-__device__ __forceinline__ LineIndex FaceLineIndex(int q, int f) {
-  const int dir = (2118 >> (2 * f)) & 3;  // {2, 1, 0, 1, 0, 2}
-  const int side = (0x2C >> f) & 1;       // {0, 0, 1, 1, 0, 1}
-  LineIndex li;
-  li.dir = dir;
-  li.side = side;
-  li.a_mul = dir == 0 ? q : q * q;
-  li.b_mul = dir == 2 ? q : 1;
-  li.stride = dir == 0 ? q * q : (dir == 1 ? q : 1);
-  return li;
-}
-// The same line in the skewed LDS layout of Shape<Q>.
-template <int Q>
-__device__ __forceinline__ LineIndex SkewedLineIndex(int f) {
-  constexpr int SI = Q * (Q + 1), SJ = Q + 1;
-  const int dir = (2118 >> (2 * f)) & 3;
-  LineIndex li;
-  li.dir = dir;
-  li.side = (0x2C >> f) & 1;
-  li.a_mul = dir == 0 ? SJ : SI;
-  li.b_mul = dir == 2 ? SJ : 1;
-  li.stride = dir == 0 ? SI : (dir == 1 ? SJ : 1);
-  return li;
-}
-
-// Extrapolation to a line end: a fixed-order fused chain so that both cells of
-// a face evaluate a given trace with identical rounding.
-template <int Q>
-__device__ __forceinline__ double Extrapolate(const double *w, const double *v) {
-  double acc = w[0] * v[0];
-#pragma unroll
-  for (int m = 1; m < Q; ++m) acc = __fma_rn(w[m], v[m], acc);
-  return acc;
-}
-
-__device__ __forceinline__ int64_t LogicalBlock(int32_t remap) {
-  int64_t b = blockIdx.x;
-  if (!remap) return b;
-  // Blocks are dispatched round-robin over the 8 XCDs; give each XCD one
-  // contiguous range of cells so neighbour traces hit its own L2.
-  int64_t n = gridDim.x, per = n / 8, rem = n % 8;
-  int64_t xcd = b % 8, idx = b / 8;
-  return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
-}
-
-// Column d of the local flux F~ = F J* at one solution point.
-template <bool kDiag>
-__device__ __forceinline__ void LocalFluxColumn(State const &c, State const &p, const double *js, int d,
-                                                double *out) {
-  const double rh = c.v[4] + p.v[4];
-  if constexpr (kDiag) {
-    const double vel = p.v[1 + d], s = js[4 * d];
-    out[0] = c.v[1 + d] * s;
-    out[1] = (c.v[1] * vel + (d == 0 ? p.v[4] : 0.0)) * s;
-    out[2] = (c.v[2] * vel + (d == 1 ? p.v[4] : 0.0)) * s;
-    out[3] = (c.v[3] * vel + (d == 2 ? p.v[4] : 0.0)) * s;
-    out[4] = rh * vel * s;
-  } else {
-#pragma unroll
-    for (int k = 0; k < 5; ++k) out[k] = 0;
-#pragma unroll
-    for (int e = 0; e < 3; ++e) {
-      const double vel = p.v[1 + e], s = js[3 * e + d];
-      out[0] += c.v[1 + e] * s;
-      out[1] += (c.v[1] * vel + (e == 0 ? p.v[4] : 0.0)) * s;
-      out[2] += (c.v[2] * vel + (e == 1 ? p.v[4] : 0.0)) * s;
-      out[3] += (c.v[3] * vel + (e == 2 ? p.v[4] : 0.0)) * s;
-      out[4] += rh * vel * s;
-    }
-  }
-}
 
 // kMixed: the F~ columns cross LDS as FP32 deviations from the cell's first
 // solution point and are contracted with FP32 FMAs (each row of the
@@ -460,29 +393,9 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
       const double *g = sg + k * FPAD;
-      double corr = g[0 * NFP + ni * Q + nj] * lt.gp[0][nk]
-                  + g[5 * NFP + ni * Q + nj] * lt.gp[1][nk]
-                  + g[1 * NFP + ni * Q + nk] * lt.gp[0][nj]
-                  + g[3 * NFP + ni * Q + nk] * lt.gp[1][nj]
-                  + g[4 * NFP + nj * Q + nk] * lt.gp[0][ni]
-                  + g[2 * NFP + nj * Q + nk] * lt.gp[1][ni];
+      const double corr = fr::LiftedGaps<Q>(g, lt, ni, nj, nk);
       const double res = (-div[k] - corr) * inv_det;
-      const double us = su[k * PAD + spos];
-      double out;
-      if (a.stage == 0) {
-        out = res;
-      } else {
-        // temporal/rk.hpp:31-49 with ode.hpp:121-128: residual *= dt; u += residual
-        const double next = AddRn(us, MulRn(res, dt));
-        if (a.stage == 1) {
-          out = next;
-        } else if (a.stage == 2) {
-          out = AddRn(MulRn(u0v[k], 3.0), next) / 4.0;
-        } else {
-          out = AddRn(MulRn(next, 2.0), u0v[k]) / 3.0;
-        }
-      }
-      outv[k] = out;
+      outv[k] = fr::RkOutput(a.stage, res, su[k * PAD + spos], u0v[k], dt);
     }
     // one basic block of stores: merged into 16-byte stores of the node record
 #pragma unroll
