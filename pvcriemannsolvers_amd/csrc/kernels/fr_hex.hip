@@ -475,13 +475,15 @@ template <> struct TileOf<4, 1> { static constexpr int kCells = 2; };
 template <> struct TileOf<4, 2> { static constexpr int kCells = 1; };
 
 template <int Q, int V>
-constexpr int TileLds() { return Shape<Q, TileOf<Q, V>::kCells>::kLdsDoubles; }
+constexpr int TileLds() { return Shape<Q, TileOf<Q, V>::kCells, 256, (Q >= 3)>::kLdsDoubles; }
 
 // One out-of-line body per (Q, tile): each is register-allocated on its own
-// instead of the batched kernel carrying the union of nine inlined bodies.
+// within the kernel's 3-waves-per-SIMD budget, where nine inlined bodies
+// would share one allocation and spill.  Q >= 3 bodies pass all three F~
+// columns through LDS in one round, as the single-mesh launch does.
 template <int Q, int V, int kRiemann, bool kMixed>
 __device__ __noinline__ void BatchBody(const FrHexBatchGroup *g, int64_t block, double *lds) {
-  FrHexStageBody<Q, TileOf<Q, V>::kCells, 256, kRiemann, true, kMixed>(g->args, g->lt, block, lds);
+  FrHexStageBody<Q, TileOf<Q, V>::kCells, 256, kRiemann, true, kMixed, (Q >= 3), 1>(g->args, g->lt, block, lds);
 }
 
 template <int Q, int kRiemann, bool kMixed>
@@ -499,7 +501,7 @@ __device__ __forceinline__ void BatchItem(const FrHexBatchGroup &g, int64_t bloc
 // cells -- stay in one XCD's L2.  LDS is sized at launch for the largest tile
 // in use.
 template <int kRiemann, bool kMixed>
-__global__ __launch_bounds__(256) void FrHexBatchKernel(FrHexBatchArgs b) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void FrHexBatchKernel(FrHexBatchArgs b) {
   extern __shared__ double lds[];
   const int64_t w = blockIdx.x, nw = gridDim.x;
   const int64_t begin = w * b.n_items / nw, end = (w + 1) * b.n_items / nw;
