@@ -207,6 +207,26 @@ typedef void (*mcfd_bc_function)(int64_t n, const double *xyz, double t, double 
 int mcfd_boundary_points(const mcfd_solver *h, int32_t tag, int64_t *n_points, double *xyz);
 int mcfd_set_boundary_function(mcfd_solver *h, int32_t tag, mcfd_bc_function f, void *user);
 
+/* limiter::Reconstruct(part, limiter) (limiter/reconstruct.hpp:22-54): one
+ * pass of the handle's limiter over the current solution, as the reference's
+ * driver runs after Approximate (twice on tetrahedra,
+ * demo/euler/sourceless.cpp:139-147).  MCFD_ESTATE without a limiter. */
+int mcfd_limiter_reconstruct(mcfd_solver *h);
+
+/* Start-up limiters of nodal (FR_GENERAL) solutions, applied to the current
+ * solution:
+ *   AVERAGE  limiter/average.hpp:11-28: every node of a cell takes the cell's
+ *            average (the global-weight quadrature over its solution points);
+ *   MAJORITY limiter/majority.hpp:14-42 (degree >= 2, else no change): every
+ *            node takes, per component, the value that more than half of the
+ *            cell's nodes hold; MCFD_EINVAL, and no change, where a component
+ *            has none (the reference asserts).
+ * MCFD_ENOTSUP for DG_GENERAL handles (the reference's start-up limiters need
+ * nodal values). */
+#define MCFD_STARTUP_AVERAGE 0
+#define MCFD_STARTUP_MAJORITY 1
+int mcfd_startup_limiter(mcfd_solver *h, int32_t kind);
+
 /* Troubled-cell flags of the last limiter application (1 = reconstructed),
  * one byte per cell, and their count; limiter/reconstruct.hpp:29-34.  Either
  * output may be null. */

@@ -226,6 +226,9 @@ void oracle_fr_set_boundary_states(void *h, int64_t n, double const *states) {
   SetPointStates(static_cast<FrHandle *>(h)->scheme.get(), n, states);
 }
 
+int oracle_fr_startup_limiter(void *h, double *u, int kind) {
+  return Guard([&] { static_cast<FrHandle *>(h)->scheme->StartupLimiter(u, kind); });
+}
 // Plants a known mistake in the FR restatement (FrScheme::mutation), so that
 // tests can show which pins catch it.
 void oracle_fr_set_mutation(void *h, int mutation) { static_cast<FrHandle *>(h)->scheme->mutation = mutation; }

@@ -12,6 +12,7 @@
 #ifndef ORACLE_FR_HPP_
 #define ORACLE_FR_HPP_
 
+#include <unordered_map>
 #include <vector>
 
 #include "core.hpp"
@@ -270,6 +271,36 @@ struct FrScheme {
         double s = 1.0 / det[c * n + ijk];
         for (int v = 0; v < 5; ++v) r[(c * n + ijk) * 5 + v] *= s;
       }
+  }
+
+  // limiter/average.hpp:11-28 (kind 0): every node takes the cell average
+  // sum_q u_q w_q / volume (polynomial/hexahedron.hpp:626-632);
+  // limiter/majority.hpp:14-42 (kind 1, degree >= 2): every node takes, per
+  // component, the first value whose running count exceeds N / 2.
+  void StartupLimiter(double *u, int kind) const {
+    if (kind == 1 && q - 1 < 2) return;
+    for (size_t c = 0; c < mesh->cells.size(); ++c) {
+      auto const &cell = mesh->cells[c];
+      double *uc = u + c * n * 5;
+      Val5 value{};
+      if (kind == 0) {
+        for (int i = 0; i < n; ++i)
+          for (int k = 0; k < 5; ++k) value[k] += uc[i * 5 + k] * cell.qw[i];
+        for (int k = 0; k < 5; ++k) value[k] /= cell.volume;
+      } else {
+        for (int k = 0; k < 5; ++k) {
+          std::unordered_map<double, int> count;
+          bool found = false;
+          for (int i = 0; i < n && !found; ++i) {
+            double v = uc[i * 5 + k];
+            if (++count[v] * 2 > n) { value[k] = v; found = true; }
+          }
+          if (!found) throw std::runtime_error("majority: no value held by more than half of the nodes");
+        }
+      }
+      for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 5; ++k) uc[i * 5 + k] = value[k];
+    }
   }
 
   // fem.hpp:417-435 with an explicit CFL number: every node is sampled.

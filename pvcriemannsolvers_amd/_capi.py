@@ -26,6 +26,7 @@ BC_SUBSONIC_INLET, BC_SUBSONIC_OUTLET = 4, 5
 FLUX_UPWIND, FLUX_INVISCID_WALL, FLUX_SUPERSONIC_OUTLET, FLUX_SUBSONIC_INLET, FLUX_SUBSONIC_OUTLET = 0, 1, 2, 3, 4
 FLUX_FORM_FR, FLUX_FORM_DG = 0, 1
 NOALLOC_MAX_POINTS = 16384
+STARTUP_AVERAGE, STARTUP_MAJORITY = 0, 1
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _i64p = ctypes.POINTER(ctypes.c_int64)
@@ -71,6 +72,8 @@ SIGNATURES = {
     "mcfd_face_count": [ctypes.c_void_p, _i64p, _i64p],
     "mcfd_faces": [ctypes.c_void_p, _i64p, _i64p, _i32p, _i32p, _i32p],
     "mcfd_limiter_flags": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint8), _i64p],
+    "mcfd_limiter_reconstruct": [ctypes.c_void_p],
+    "mcfd_startup_limiter": [ctypes.c_void_p, ctypes.c_int32],
     "mcfd_boundary_points": [ctypes.c_void_p, ctypes.c_int32, _i64p, _dp],
     "mcfd_set_boundary_function": [ctypes.c_void_p, ctypes.c_int32, BC_FUNCTION, ctypes.c_void_p],
     "mcfd_face_fluxes": [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, _dp, _dp, _dp,

@@ -60,8 +60,8 @@ void FrHexGroup::Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &schem
 
   // line tables
   x1d.resize(q);
-  std::vector<double> w(q);
-  GaussRule(scheme.nodes == MCFD_NODES_LOBATTO, q, x1d.data(), w.data());
+  w1d.resize(q);
+  GaussRule(scheme.nodes == MCFD_NODES_LOBATTO, q, x1d.data(), w1d.data());
   std::memset(&lt, 0, sizeof(lt));
   for (int i = 0; i < q; ++i)
     for (int m = 0; m < q; ++m) lt.d[i][m] = LagrangeDerivative(q, x1d.data(), m, x1d[i]);

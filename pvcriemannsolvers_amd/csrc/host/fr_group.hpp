@@ -20,7 +20,7 @@ struct FrHexGroup {
   int64_t n_cells = 0;
   HexConnectivity conn;
   LineTables lt;
-  std::vector<double> x1d;
+  std::vector<double> x1d, w1d;  // 1-D solution points and their quadrature weights
   std::vector<std::array<double, 3>> corner0;  // node 0 of each cell
   // host copies uploaded by Upload()
   std::vector<int32_t> h_nbr, h_finfo, h_geo_id, h_bc_kind, h_cell_case, h_tile_order;

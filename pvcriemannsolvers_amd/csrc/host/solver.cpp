@@ -12,6 +12,7 @@
 #include "common.hpp"
 #include "dg_group.hpp"
 #include "fr_group.hpp"
+#include "../kernels/fr_startup.hpp"
 
 namespace mcfd {
 
@@ -292,6 +293,53 @@ int mcfd_set_boundary_function(mcfd_solver *h, int32_t tag, mcfd_bc_function f, 
   MCFD_REQUIRE_DEVICE(h);
   if (tag < 0 || tag >= (int32_t)h->bct.fns.size()) return Fail(MCFD_EINVAL, "boundary tag out of range");
   return Guard([&] { h->bct.SetFunction(tag, f, user); });
+}
+
+int mcfd_limiter_reconstruct(mcfd_solver *h) {
+  MCFD_REQUIRE_DEVICE(h);
+  if (!h->dg || !h->modal.limiter) return Fail(MCFD_ESTATE, "the handle has no limiter");
+  return Guard([&] {
+    h->dt_fresh = false;
+    h->modal.Limit(h->modal.u0.ptr, h->modal.ct0.ptr, h->stream);
+    Check(hipStreamSynchronize(h->stream), "limiter sync");
+  });
+}
+
+int mcfd_startup_limiter(mcfd_solver *h, int32_t kind) {
+  MCFD_REQUIRE_DEVICE(h);
+  if (kind != MCFD_STARTUP_AVERAGE && kind != MCFD_STARTUP_MAJORITY) return Fail(MCFD_EINVAL, "unknown start-up limiter");
+  if (h->dg) return Fail(MCFD_ENOTSUP, "the start-up limiters act on nodal (FR_GENERAL) solutions");
+  auto &g = h->group;
+  // average.hpp:13 and majority.hpp:18 leave low degrees alone
+  if (kind == MCFD_STARTUP_MAJORITY && g.degree < 2) return MCFD_OK;
+  return Guard([&] {
+    h->dt_fresh = false;
+    const int q = g.q;
+    if (kind == MCFD_STARTUP_AVERAGE) {
+      const auto &w = g.w1d;
+      std::vector<double> wl;  // local weights in node order (integrator/hexahedron.hpp:66-93)
+      for (int i = 0; i < q; ++i)
+        for (int j = 0; j < q; ++j)
+          for (int k = 0; k < q; ++k) wl.push_back(w[i] * w[j] * w[k]);
+      mcfd::DeviceArray<double> d_wl, d_det;
+      d_wl.Upload(wl, h->stream);
+      d_det.Upload(g.conn.det, h->stream);
+      Check(mcfd::LaunchFrAverage(g.u0.ptr, d_wl.ptr, d_det.ptr, g.n_cells, g.np, h->stream), "average launch");
+      Check(hipStreamSynchronize(h->stream), "average sync");
+      return;
+    }
+    mcfd::DeviceArray<double> value;
+    mcfd::DeviceArray<int32_t> failed;
+    value.Allocate(size_t(g.n_cells) * 5);
+    failed.Upload(std::vector<int32_t>{ 0 }, h->stream);
+    Check(mcfd::LaunchFrMajority(g.u0.ptr, value.ptr, failed.ptr, g.n_cells, g.np, h->stream), "majority launch");
+    int32_t bad = 0;
+    Check(hipMemcpyAsync(&bad, failed.ptr, sizeof(bad), hipMemcpyDeviceToHost, h->stream), "majority flag");
+    Check(hipStreamSynchronize(h->stream), "majority sync");
+    if (bad) throw mcfd::Failure{ MCFD_EINVAL, "majority limiter: a cell component has no value held by more than half of its nodes" };
+    Check(mcfd::LaunchFrSetNodes(g.u0.ptr, value.ptr, g.n_cells, g.np, h->stream), "majority set");
+    Check(hipStreamSynchronize(h->stream), "majority sync");
+  });
 }
 
 int mcfd_limiter_flags(mcfd_solver *h, uint8_t *flags, int64_t *n_troubled) {

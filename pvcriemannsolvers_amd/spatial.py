@@ -142,6 +142,15 @@ class FiniteElement(System):
         self._bc_callbacks[tag] = cb  # kept alive while the library may call it
         C.check(C.load().mcfd_set_boundary_function(self._h, int(tag), cb, None))
 
+    def limiter_reconstruct(self):
+        """limiter::Reconstruct (limiter/reconstruct.hpp:22-54): one limiter pass over the current solution."""
+        C.check(C.load().mcfd_limiter_reconstruct(self._h))
+
+    def startup_limiter(self, kind):
+        """Start-up limiter of a nodal solution: "average" (limiter/average.hpp) or "majority" (majority.hpp)."""
+        C.check(C.load().mcfd_startup_limiter(self._h, {"average": C.STARTUP_AVERAGE,
+                                                         "majority": C.STARTUP_MAJORITY}[kind]))
+
     def limiter_flags(self):
         """Troubled-cell flags of the last limiter pass (limiter/reconstruct.hpp:29-34)."""
         flags = np.zeros(self.n_cells, np.uint8)

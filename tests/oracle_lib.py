@@ -285,6 +285,14 @@ class FrOracle(_BoundaryValues):
         _check(load().oracle_fr_residual(self.h, _ptr(u, _dp), _ptr(r, _dp)))
         return r
 
+    def startup_limiter(self, u, kind):
+        """limiter/average.hpp ("average") or limiter/majority.hpp ("majority") on a nodal column."""
+        lib = load()
+        lib.oracle_fr_startup_limiter.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int]
+        u = f64(u).copy()
+        _check(lib.oracle_fr_startup_limiter(self.h, _ptr(u, _dp), {"average": 0, "majority": 1}[kind]))
+        return u
+
     def set_mutation(self, mutation: int):
         """Plants a known mistake in the restatement (0 = none); see FrScheme::mutation."""
         lib = load()

@@ -169,3 +169,58 @@ def test_time_dependent_smart_boundary(scheme):
         u = limit(u)
     assert rel(g.get_residual_column(), o.residual(u)) < 1e-12
     g.close()
+
+
+# ---------------------------------------------------------------- start-up limiters
+@pytest.mark.parametrize("degree", [1, 2, 3])
+def test_startup_limiters_match_oracle(degree):
+    """limiter/average.hpp and limiter/majority.hpp on the device: the average
+    to 1e-14 (a different summation of det J), the majority bit for bit."""
+    mesh = meshgen.box_hex(4, 3, 2, hi=(1.0, 0.6, 0.4), shuffle_seed=5)
+    bcs = [("inviscid_wall", None)] * 6
+    o = ol.FrOracle(mesh, q=degree + 1, riemann=1, bcs=oracle_bcs(bcs))
+    g = spatial.FiniteElement(mesh, degree=degree, riemann="hllc", bcs=bcs)
+    x = o.node_coords()
+    rng = np.random.default_rng(degree)
+    u = 1.0 + 0.3 * rng.random(x.shape[:-1] + (5,))
+    g.set_solution_column(u.reshape(-1))
+    g.startup_limiter("average")
+    assert rel(g.get_solution_column(), o.startup_limiter(u.reshape(-1), "average")) < 1e-14
+    n = x.shape[1]
+    base = rng.random((x.shape[0], 5))
+    u = np.repeat(base[:, None, :], n, axis=1)
+    for c in range(x.shape[0]):
+        idx = rng.choice(n, (n - 1) // 2, replace=False)
+        u[c, idx, :] += rng.random((len(idx), 5))
+    g.set_solution_column(u.reshape(-1))
+    g.startup_limiter("majority")
+    np.testing.assert_array_equal(g.get_solution_column(), o.startup_limiter(u.reshape(-1), "majority"))
+    if degree >= 2:
+        u[0, : n // 2 + 1, 1] = np.arange(n // 2 + 1.0)  # no majority in one component
+        g.set_solution_column(u.reshape(-1))
+        with pytest.raises(Exception):
+            g.startup_limiter("majority")
+        np.testing.assert_array_equal(g.get_solution_column(), u.reshape(-1))  # left unchanged
+    g.close()
+
+
+def test_limiter_reconstruct_and_dg_startup_refusal():
+    """limiter::Reconstruct once more on the current state (the reference's
+    start-up for tetrahedra runs it twice) equals a second oracle pass; the
+    nodal start-up limiters refuse modal DG."""
+    mesh = meshgen.box_tet(4, 4, 1, hi=(1.0, 1.0, 0.25), jitter=0.1, seed=6)
+    o = ol.DgOracle(mesh, degree=2, riemann=1, bcs=[(1, [0] * 5)] * 6)
+    o.set_limiter(1)
+    g = spatial.FiniteElement(mesh, degree=2, scheme="dg", riemann="hllc", bcs=[("inviscid_wall", None)] * 6,
+                              limiter="weno")
+    x = g.sample_points()
+    vals = np.where((x[..., 0] < 0.5)[..., None], cons(1.0, (0, 0, 0), 1.0), cons(0.2, (0, 0, 0), 0.1))
+    g.approximate(vals)
+    u, _ = o.limit(o.approximate(vals))
+    g.limiter_reconstruct()
+    u2, flags = o.limit(u)
+    assert rel(g.get_solution_column(), u2) < 1e-12
+    np.testing.assert_array_equal(g.limiter_flags(), flags)
+    with pytest.raises(Exception):
+        g.startup_limiter("average")
+    g.close()

@@ -139,3 +139,42 @@ def test_timed_rk3_equals_restatement_step(scheme):
     got = ol.rk3_timed(o, u, 0.0, dt, funcs, constants, limit)
     o.set_boundary_states(None)
     np.testing.assert_array_equal(got, want)
+
+
+# ---------------------------------------------------------------- start-up limiters
+def test_average_limiter_conserves_and_flattens():
+    """limiter/average.hpp: each cell becomes constant at its weighted average,
+    so the quadrature integral of every component is unchanged."""
+    mesh = meshgen.box_hex(3, 2, 2, hi=(1.0, 0.6, 0.4), shuffle_seed=2)
+    o = ol.FrOracle(mesh, q=4, riemann=1, bcs=[(1, [0] * 5)] * 6)
+    x = o.node_coords()
+    rng = np.random.default_rng(1)
+    u = (1.0 + 0.3 * rng.random(x.shape[:-1] + (5,)))
+    v = o.startup_limiter(u.reshape(-1), "average").reshape(u.shape)
+    w = o.node_weights()
+    assert np.allclose(np.einsum("cn,cnk->ck", w, v), np.einsum("cn,cnk->ck", w, u), rtol=1e-14, atol=0)
+    assert np.all(v == v[:, :1, :])
+    np.testing.assert_allclose(o.startup_limiter(v.reshape(-1), "average").reshape(u.shape), v, rtol=1e-14)
+
+
+def test_majority_limiter():
+    """limiter/majority.hpp: the value more than half of a cell's nodes hold
+    wins, per component; degree 1 is left alone; no majority is an error."""
+    mesh = meshgen.box_hex(2, 2, 1, shuffle_seed=3)
+    o = ol.FrOracle(mesh, q=3, riemann=1, bcs=[(1, [0] * 5)] * 6)
+    n = 27
+    rng = np.random.default_rng(4)
+    base = rng.random((4, 5))
+    u = np.repeat(base[:, None, :], n, axis=1)
+    for c in range(4):  # 13 of 27 nodes perturbed, per component
+        for k in range(5):
+            idx = rng.choice(n, 13, replace=False)
+            u[c, idx, k] += rng.random(13)
+    v = o.startup_limiter(u.reshape(-1), "majority").reshape(u.shape)
+    np.testing.assert_array_equal(v, np.repeat(base[:, None, :], n, axis=1))
+    u[0, :14, 2] = np.arange(14.0)  # 13 equal values left: no majority
+    with pytest.raises(RuntimeError):
+        o.startup_limiter(u.reshape(-1), "majority")
+    o1 = ol.FrOracle(mesh, q=2, riemann=1, bcs=[(1, [0] * 5)] * 6)
+    w = rng.random(4 * 8 * 5)
+    np.testing.assert_array_equal(o1.startup_limiter(w, "majority"), w)
