@@ -48,12 +48,13 @@ class FiniteElement : public temporal::System<double> {
   FiniteElement(mcfd_mesh_desc const &mesh, mcfd_scheme_desc const &scheme, std::vector<mcfd_bc_desc> const &bcs) {
     Check(mcfd_create(&mesh, &scheme, bcs.data(), (int32_t)bcs.size(), &h_));
     Check(mcfd_sizes(h_, &n_cells_, &n_per_cell_, &size_));
+    degree_ = scheme.degree;
   }
   FiniteElement(FiniteElement const &) = delete;
   FiniteElement &operator=(FiniteElement const &) = delete;
   FiniteElement(FiniteElement &&that) noexcept
       : h_(std::exchange(that.h_, nullptr)), n_cells_(that.n_cells_), n_per_cell_(that.n_per_cell_),
-        size_(that.size_) {}
+        size_(that.size_), degree_(that.degree_) {}
   ~FiniteElement() override {
     if (h_) mcfd_destroy(h_);
   }
@@ -85,6 +86,19 @@ class FiniteElement : public temporal::System<double> {
     s.limiter = all_cells ? Limiter::kAll : Limiter::kFlagged;
     s.weno_w0 = w0;
     s.weno_eps = eps;
+    return FiniteElement(mesh, s, bcs);
+  }
+  // WithLimiter(limiter_ptr, ...) of with_limiter.hpp:40-43: the limiter
+  // object's w0 and eps and its own troubled-cell rule (Limiter::kDefault)
+  template <class Riemann, class Limiter>
+  static FiniteElement WithLimiter(Limiter const *limiter_ptr, mcfd_mesh_desc const &mesh, int degree,
+                                   std::vector<mcfd_bc_desc> const &bcs) {
+    mcfd_scheme_desc s = Scheme(MCFD_SCHEME_DG_GENERAL, degree, Riemann::kId);
+    s.limiter = limiter_ptr ? Limiter::kDefault : MCFD_LIMITER_NONE;
+    if (limiter_ptr) {
+      s.weno_w0 = limiter_ptr->w0();
+      s.weno_eps = limiter_ptr->eps();
+    }
     return FiniteElement(mesh, s, bcs);
   }
 
@@ -125,6 +139,7 @@ class FiniteElement : public temporal::System<double> {
   void Rk3Update(double t_curr, double dt) { Check(mcfd_rk3_update(h_, t_curr, dt)); }
 
   int64_t CountCells() const { return n_cells_; }
+  int Degree() const { return degree_; }
   int32_t UnknownsPerCell() const { return n_per_cell_; }
   int64_t ColumnSize() const { return size_; }
   mcfd_solver *handle() const { return h_; }
@@ -148,6 +163,7 @@ class FiniteElement : public temporal::System<double> {
   int64_t n_cells_ = 0;
   int32_t n_per_cell_ = 0;
   int64_t size_ = 0;
+  int degree_ = 0;
 };
 
 }  // namespace mi355x

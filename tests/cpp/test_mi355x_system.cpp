@@ -3,11 +3,16 @@
 // the columns of the fused device step (RungeKutta<3>::Update of
 // temporal/rk.hpp:31-49 with the same operation order), for nodal FR, modal
 // DG and modal DG with the WENO limiter; and the time step of a column read
-// back and set again is unchanged.
+// back and set again is unchanged.  The limiter objects' form of WithLimiter
+// equals its identifier form, and the start-up limiters average::Reconstruct
+// and majority::Reconstruct leave every cell of a nodal solution constant.
 #include <cmath>
 #include <cstdio>
 #include <vector>
 
+#include "mini/limiter/average.hpp"
+#include "mini/limiter/majority.hpp"
+#include "mini/limiter/reconstruct.hpp"
 #include "mini/limiter/weno.hpp"
 #include "mini/riemann/euler.hpp"
 #include "mini/spatial/mi355x.hpp"
@@ -112,11 +117,23 @@ bool Compare(char const *name, FiniteElement &system_path, FiniteElement &fused_
   return ok;
 }
 
+// every cell of the column constant per component after a start-up limiter
+bool CellsConstant(FiniteElement const &fe) {
+  const Column c = fe.GetSolutionColumn();
+  const int n = fe.UnknownsPerCell();  // solution points per cell
+  bool ok = true;
+  for (int64_t cell = 0; cell < fe.CountCells(); ++cell)
+    for (int i = 1; i < n; ++i)
+      for (int k = 0; k < 5; ++k) ok &= c[(cell * n + i) * 5 + k] == c[(cell * n) * 5 + k];
+  return ok;
+}
+
 }  // namespace
 
 int main() {
-  using mini::riemann::euler::HartenLaxLeerContact;
-  using mini::riemann::euler::Roe;
+  using mini::riemann::euler::IdealGas;
+  using HartenLaxLeerContact = mini::riemann::rotated::Euler<mini::riemann::euler::HartenLaxLeerContact<IdealGas, 3>>;
+  using Roe = mini::riemann::rotated::Euler<mini::riemann::euler::Roe<IdealGas, 3>>;
   namespace mi = mini::spatial::mi355x;
   bool ok = true;
   try {
@@ -137,6 +154,34 @@ int main() {
       auto a = FiniteElement::WithLimiter<HartenLaxLeerContact, Lazy>(walls.desc, 2, wall_bcs);
       auto b = FiniteElement::WithLimiter<HartenLaxLeerContact, Lazy>(walls.desc, 2, wall_bcs);
       ok &= Compare("WithLimiter<dg::General<HLLC>, weno::Lazy> p=2, jump, walls 4^3", a, b, true, 3, true);
+    }
+    {
+      // weno::Lazy(w0, eps) reconstructs every cell (its IsNotSmooth is always true)
+      using Lazy = mini::limiter::weno::Lazy;
+      Lazy lazy(0.001, 1e-6);
+      auto a = FiniteElement::WithLimiter<HartenLaxLeerContact>(&lazy, walls.desc, 2, wall_bcs);
+      auto b = FiniteElement::WithLimiter<HartenLaxLeerContact, Lazy>(walls.desc, 2, wall_bcs, true);
+      a.Approximate(Values(a, true));
+      b.Approximate(Values(b, true));
+      mini::limiter::Reconstruct(&a, &lazy);
+      mini::limiter::Reconstruct(&b, &lazy);
+      const bool same = a.GetSolutionColumn() == b.GetSolutionColumn();
+      std::printf("%s WithLimiter(&lazy, ...) equals WithLimiter<.., Lazy>(.., all cells) after Reconstruct\n",
+                  same ? "PASS" : "FAIL");
+      ok &= same;
+    }
+    {
+      // start-up limiters of a nodal solution (limiter/average.hpp, majority.hpp)
+      auto fr = FiniteElement::FluxReconstruction<Roe>(walls.desc, 3, wall_bcs);
+      fr.Approximate(Values(fr, true));
+      mini::limiter::average::Reconstruct(&fr);
+      const Column averaged = fr.GetSolutionColumn();
+      const bool constant = CellsConstant(fr);
+      mini::limiter::majority::Reconstruct(&fr);
+      const bool kept = fr.GetSolutionColumn() == averaged;
+      std::printf("%s average::Reconstruct leaves cells constant (%d), majority::Reconstruct keeps them (%d)\n",
+                  constant && kept ? "PASS" : "FAIL", (int)constant, (int)kept);
+      ok &= constant && kept;
     }
     // errors of the C ABI surface as exceptions
     bool threw = false;
