@@ -178,7 +178,10 @@ void DgGroup::Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &scheme,
   // ---- faces
   std::vector<std::array<double, 3>> shift;
   if (hex) {
-    HexConnectivity conn = BuildHexConnectivity(mesh);  // checks the affine shape
+    HexConnectivity conn = BuildHexConnectivity(mesh);
+    for (auto a : conn.affine)
+      if (!a) throw Failure{ MCFD_ENOTSUP, "modal DG supports affine (parallelepiped) hexahedra only; "
+                                           "trilinear hexahedra run with FR_GENERAL" };
     n_interior = conn.n_interior;
     for (auto const &f : conn.faces) {
       faces.push_back(DgFace{ f.holder, f.sharer, f.holder_face, f.sharer_face, 0, f.tag });

@@ -31,7 +31,107 @@ int InverseOrient(int code) {
   return -1;
 }
 
+// J* = det(J) J^{-1}, the adjugate of J (rows d/dxi, d/deta, d/dzeta), row-major
+// so that F~_d = sum_e F_e g[3 e + d]; returns det(J).
+double Adjugate(const double *j, double *g) {
+  g[0] = j[4] * j[8] - j[5] * j[7]; g[1] = j[2] * j[7] - j[1] * j[8]; g[2] = j[1] * j[5] - j[2] * j[4];
+  g[3] = j[5] * j[6] - j[3] * j[8]; g[4] = j[0] * j[8] - j[2] * j[6]; g[5] = j[2] * j[3] - j[0] * j[5];
+  g[6] = j[3] * j[7] - j[4] * j[6]; g[7] = j[1] * j[6] - j[0] * j[7]; g[8] = j[0] * j[4] - j[1] * j[3];
+  return j[0] * g[0] + j[1] * g[3] + j[2] * g[6];
+}
+
+// Local coordinates of flux point (a, b) of local face f (the end of the
+// solution-point line through it, fr/general.hpp:103-205).
+void FacePointLocal(int f, const double *x1d, int a, int b, double *s) {
+  const int dir = kHexFaceNormalDir[f];
+  s[dir] = kHexFaceSide[f] ? 1.0 : -1.0;
+  if (dir == 2) { s[0] = x1d[a]; s[1] = x1d[b]; }
+  else if (dir == 1) { s[0] = x1d[a]; s[2] = x1d[b]; }
+  else { s[1] = x1d[a]; s[2] = x1d[b]; }
+}
+
 }  // namespace
+
+// Geometry of a mesh with trilinear cells (polynomial/hexahedron.hpp:172-239
+// keeps J* and det(J) per node; fr/general.hpp:103-205 the flux point's
+// J* column; the face integrator the frame of each face point):
+//   geo_node [cell][node]  J* (9), 1 / det(J) at the node;
+//   fscale [cell][face][a * Q + b]  +-|J*.col(normal dir)| at the flux point;
+//   frames [face][a * Q + b]  n, t of the face map at the point, indexed by the
+//     holder's in-face (a, b);
+//   geo [cell]  only the length cbrt(volume) / 2, volume = sum of w det(J).
+void FrHexGroup::BuildCurved(const mcfd_scheme_desc &) {
+  const int64_t nc = n_cells;
+  const int nfp = q * q;
+  diag_metric = false;
+  h_geo.assign(nc * kGeoStride, 0.0);
+  h_geo_id.resize(nc);
+  h_geo_node.assign(nc * np * kNodeGeoStride, 0.0);
+  h_fscale.assign(nc * 6 * nfp, 0.0);
+  corner0.resize(nc);
+  for (int64_t c = 0; c < nc; ++c) {
+    const double *xv = conn.vertices[c].data();
+    double volume = 0;
+    for (int i = 0; i < q; ++i)
+      for (int jj = 0; jj < q; ++jj)
+        for (int k = 0; k < q; ++k) {
+          const double s[3] = { x1d[i], x1d[jj], x1d[k] };
+          double jac[9];
+          TrilinearJacobian(xv, s, jac);
+          double *g = h_geo_node.data() + (c * np + (i * q + jj) * q + k) * kNodeGeoStride;
+          const double det = Adjugate(jac, g);
+          if (!(det > 0)) throw std::invalid_argument("inverted trilinear hexahedron");
+          g[9] = 1.0 / det;
+          volume += w1d[i] * w1d[jj] * w1d[k] * det;
+        }
+    h_geo[c * kGeoStride + 16] = std::cbrt(volume) * 0.5;  // mesh/part.hpp:230
+    h_geo_id[c] = (int32_t)c;
+    for (int f = 0; f < 6; ++f) {
+      const int dir = kHexFaceNormalDir[f];
+      for (int a = 0; a < q; ++a)
+        for (int b = 0; b < q; ++b) {
+          double s[3], jac[9], g[9];
+          FacePointLocal(f, x1d.data(), a, b, s);
+          TrilinearJacobian(xv, s, jac);
+          Adjugate(jac, g);
+          const double n = std::sqrt(g[dir] * g[dir] + g[3 + dir] * g[3 + dir] + g[6 + dir] * g[6 + dir]);
+          h_fscale[(c * 6 + f) * nfp + a * q + b] = kHexFaceSide[f] ? n : -n;
+        }
+    }
+    corner0[c] = { xv[0], xv[1], xv[2] };
+  }
+  if ((int64_t)conn.faces.size() >= (int64_t(1) << 24))
+    throw Failure{ MCFD_ENOTSUP, "too many faces for the per-point frames of a trilinear mesh" };
+  h_frames.assign(conn.faces.size() * nfp * kFrameStride, 0.0);
+  h_nbr.assign(6 * nc, INT32_MIN);
+  h_finfo.assign(6 * nc, 0);
+  for (size_t fi = 0; fi < conn.faces.size(); ++fi) {
+    const auto &face = conn.faces[fi];
+    for (int i = 0; i < q; ++i)
+      for (int j = 0; j < q; ++j) {
+        int a, b;
+        ApplyOrient(face.quad_to_holder, q, i, j, &a, &b);
+        FaceFrameAt(face.corners, x1d[i], x1d[j], h_frames.data() + (fi * nfp + a * q + b) * kFrameStride);
+      }
+    const int32_t fid = (int32_t)fi;
+    const int64_t hs = face.holder * 6 + face.holder_face;
+    if (face.sharer >= 0) {
+      const int64_t ss = face.sharer * 6 + face.sharer_face;
+      h_nbr[hs] = (int32_t)face.sharer;
+      h_finfo[hs] = (fid << 7) | (1 << 6) | (face.orient << 3) | face.sharer_face;
+      h_nbr[ss] = (int32_t)face.holder;
+      h_finfo[ss] = (fid << 7) | (InverseOrient(face.orient) << 3) | face.holder_face;
+    } else {
+      h_nbr[hs] = -(1 + face.tag);
+      h_finfo[hs] = (fid << 7) | (1 << 6);
+    }
+  }
+  for (auto v : h_nbr)
+    if (v == INT32_MIN) throw std::invalid_argument("cell face without a face record");
+  h_bface_of.assign(6 * nc, -1);
+  for (size_t f = conn.n_interior; f < conn.faces.size(); ++f)
+    h_bface_of[conn.faces[f].holder * 6 + conn.faces[f].holder_face] = (int32_t)(f - conn.n_interior);
+}
 
 void FrHexGroup::Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &scheme,
                        const std::vector<int32_t> &bc_kinds, const std::vector<double> &bc_states) {
@@ -72,8 +172,17 @@ void FrHexGroup::Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &schem
     lt.gp[1][m] = CorrectionRightDerivative(degree, scheme.c_next, x1d[m]);
   }
 
-  // geometry records, deduplicated bitwise
   const int64_t nc = mesh.n_cells;
+  curved = false;
+  for (auto a : conn.affine)
+    if (!a) curved = true;
+  if (curved) {
+    if (mixed) throw Failure{ MCFD_ENOTSUP, "the mixed-precision form supports affine hexahedra only" };
+    BuildCurved(scheme);
+    return;
+  }
+
+  // geometry records, deduplicated bitwise
   std::map<std::array<double, kGeoStride>, int32_t> geo_ids;
   h_geo.clear();
   h_geo_id.assign(nc, 0);
@@ -161,6 +270,10 @@ void FrHexGroup::BoundaryPoints(std::vector<int32_t> *tags, std::vector<double> 
         else if (dir == 1) { s[0] = x1d[ia]; s[2] = x1d[ib]; }
         else { s[1] = x1d[ia]; s[2] = x1d[ib]; }
         double *p = xyz->data() + ((b * q + ia) * q + ib) * 3;
+        if (curved) {
+          TrilinearPoint(conn.vertices[face.holder].data(), s, p);
+          continue;
+        }
         for (int d = 0; d < 3; ++d)
           p[d] = x0[d] + (s[0] + 1) * j[d] + (s[1] + 1) * j[3 + d] + (s[2] + 1) * j[6 + d];
       }
@@ -176,6 +289,10 @@ void FrHexGroup::Upload(hipStream_t stream) {
   geo_id.Upload(h_geo_id, stream);
   geo.Upload(h_geo, stream);
   frames.Upload(h_frames, stream);
+  if (curved) {
+    geo_node.Upload(h_geo_node, stream);
+    fscale.Upload(h_fscale, stream);
+  }
   bc_kind.Upload(h_bc_kind, stream);
   bc_state.Upload(h_bc_state, stream);
   if (!h_cell_case.empty()) cell_case.Upload(h_cell_case, stream);
@@ -244,6 +361,9 @@ FrHexArgs FrHexGroup::Args(int stage, double dt) const {
   a.bface_of = bface_of.ptr;
   a.diag_metric = diag_metric ? 1 : 0;
   a.mixed = mixed ? 1 : 0;
+  a.curved = curved ? 1 : 0;
+  a.geo_node = geo_node.ptr;
+  a.fscale = fscale.ptr;
   switch (stage) {
   case 0: a.us = u0.ptr; a.out = res.ptr; break;
   case 1: a.us = u0.ptr; a.out = u1.ptr; break;
@@ -260,11 +380,35 @@ void FrHexGroup::SamplePoints(double *xyz) const {
     for (int i = 0; i < q; ++i)
       for (int jj = 0; jj < q; ++jj)
         for (int k = 0; k < q; ++k) {
-          double s[3] = { x1d[i] + 1, x1d[jj] + 1, x1d[k] + 1 };
           double *p = xyz + ((c * np) + (i * q + jj) * q + k) * 3;
+          if (curved) {
+            const double l[3] = { x1d[i], x1d[jj], x1d[k] };
+            TrilinearPoint(conn.vertices[c].data(), l, p);
+            continue;
+          }
+          double s[3] = { x1d[i] + 1, x1d[jj] + 1, x1d[k] + 1 };
           for (int d = 0; d < 3; ++d) p[d] = x0[d] + s[0] * j[d] + s[1] * j[3 + d] + s[2] * j[6 + d];
         }
   }
+}
+
+std::vector<double> FrHexGroup::NodeDeterminants() const {
+  std::vector<double> det(size_t(n_cells) * np);
+  for (int64_t c = 0; c < n_cells; ++c)
+    for (int i = 0; i < q; ++i)
+      for (int jj = 0; jj < q; ++jj)
+        for (int k = 0; k < q; ++k) {
+          double &out = det[c * np + (i * q + jj) * q + k];
+          if (!curved) {
+            out = conn.det[c];
+            continue;
+          }
+          const double s[3] = { x1d[i], x1d[jj], x1d[k] };
+          double jac[9], g[9];
+          TrilinearJacobian(conn.vertices[c].data(), s, jac);
+          out = Adjugate(jac, g);
+        }
+  return det;
 }
 
 void FrHexGroup::Faces(int64_t *holder, int64_t *sharer, int32_t *holder_face, int32_t *sharer_face,

@@ -1,6 +1,10 @@
-// One mesh discretised by nodal FR on affine hexahedra: host connectivity,
-// line tables, geometry/frame records and the device arrays the stage kernel
+// One mesh discretised by nodal FR on hexahedra: host connectivity, line
+// tables, geometry/frame records and the device arrays the stage kernel
 // reads.  A solver handle owns one group; a batch owns one per degree.
+// Meshes of affine cells keep one deduplicated geometry record per cell and
+// one frame per face; a mesh with a trilinear cell ("curved") keeps the
+// metric of every solution point, the signed scale of every cell flux point
+// and the frame of every face flux point.
 #ifndef MINICFD_HOST_FR_GROUP_HPP_
 #define MINICFD_HOST_FR_GROUP_HPP_
 
@@ -16,7 +20,7 @@ namespace mcfd {
 
 struct FrHexGroup {
   int q = 0, np = 0, degree = 0, riemann = 0;
-  bool mixed = false, diag_metric = false;
+  bool mixed = false, diag_metric = false, curved = false;
   int64_t n_cells = 0;
   HexConnectivity conn;
   LineTables lt;
@@ -25,6 +29,9 @@ struct FrHexGroup {
   // host copies uploaded by Upload()
   std::vector<int32_t> h_nbr, h_finfo, h_geo_id, h_bc_kind, h_cell_case, h_tile_order;
   std::vector<double> h_geo, h_frames, h_bc_state;
+  // curved meshes: [cell][node][kNodeGeoStride] and [cell][6][Q * Q]
+  std::vector<double> h_geo_node, h_fscale;
+  DeviceArray<double> geo_node, fscale;
   DeviceArray<double> u0, u1, u2, res;
   DeviceArray<int32_t> nbr, finfo, geo_id, bc_kind, cell_case, tile_order;
   DeviceArray<double> geo, frames, bc_state, case_dt;
@@ -39,6 +46,7 @@ struct FrHexGroup {
   // Host-side construction; bc_kind/bc_state index the mesh's boundary tags.
   void Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &scheme,
              const std::vector<int32_t> &bc_kinds, const std::vector<double> &bc_states);
+  void BuildCurved(const mcfd_scheme_desc &scheme);
   // Order in which the stage kernel sweeps its tiles of `tile_cells`
   // consecutive cells: Morton order of the tile centres, so that the tiles in
   // flight on one XCD are spatial neighbours and their face-trace gathers hit
@@ -51,6 +59,8 @@ struct FrHexGroup {
   // Kernel arguments of one pass: stage 0 residual, 1..3 SSP-RK3 stages.
   FrHexArgs Args(int stage, double dt) const;
   void SamplePoints(double *xyz) const;
+  // det(J) at every solution point, [cell][node] (the start-up average's weights)
+  std::vector<double> NodeDeterminants() const;
   // Tag and flux points of every boundary face ([boundary face][a * Q + b][3],
   // (a, b) the in-face indices of the holder's local face).
   void BoundaryPoints(std::vector<int32_t> *tags, std::vector<double> *xyz) const;

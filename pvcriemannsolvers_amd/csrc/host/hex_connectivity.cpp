@@ -72,6 +72,37 @@ struct FaceKey {
 
 }  // namespace
 
+// coordinate/hexahedron.hpp:397-526 (Hexahedron8): x = sum_i N_i(xi) x_i with
+// N_i = (1 + a_i xi)(1 + b_i eta)(1 + c_i zeta) / 8 over the CGNS corners.
+void TrilinearPoint(const double *xv, const double *xi, double *x) {
+  x[0] = x[1] = x[2] = 0;
+  for (int i = 0; i < 8; ++i) {
+    const double v = (1 + kCorner[i][0] * xi[0]) * (1 + kCorner[i][1] * xi[1]) * (1 + kCorner[i][2] * xi[2]) / 8;
+    for (int d = 0; d < 3; ++d) x[d] += xv[3 * i + d] * v;
+  }
+}
+
+void TrilinearJacobian(const double *xv, const double *xi, double *j) {
+  for (int e = 0; e < 9; ++e) j[e] = 0;
+  for (int i = 0; i < 8; ++i) {
+    const double fx = 1 + kCorner[i][0] * xi[0], fy = 1 + kCorner[i][1] * xi[1], fz = 1 + kCorner[i][2] * xi[2];
+    const double g[3] = { kCorner[i][0] * fy * fz / 8, kCorner[i][1] * fz * fx / 8, kCorner[i][2] * fx * fy / 8 };
+    for (int r = 0; r < 3; ++r)
+      for (int d = 0; d < 3; ++d) j[3 * r + d] += g[r] * xv[3 * i + d];
+  }
+}
+
+void FaceFrameAt(const double *xq, double a, double b, double *frame) {
+  Vec ra{ 0, 0, 0 }, rb{ 0, 0, 0 };
+  for (int i = 0; i < 4; ++i)
+    for (int d = 0; d < 3; ++d) {
+      ra[d] += kQuadCorner[i][0] * (1 + kQuadCorner[i][1] * b) / 4 * xq[3 * i + d];
+      rb[d] += kQuadCorner[i][1] * (1 + kQuadCorner[i][0] * a) / 4 * xq[3 * i + d];
+    }
+  Vec n = Unit(Cross(ra, rb)), t = Unit(ra);
+  for (int d = 0; d < 3; ++d) { frame[d] = n[d]; frame[3 + d] = t[d]; }
+}
+
 HexConnectivity BuildHexConnectivity(const mcfd_mesh_desc &mesh) {
   if (mesh.cell_type != MCFD_HEXA_8) throw std::invalid_argument("hexahedral mesh expected");
   if (mesh.n_cells <= 0 || mesh.n_cells > (int64_t(1) << 30))
@@ -87,21 +118,36 @@ HexConnectivity BuildHexConnectivity(const mcfd_mesh_desc &mesh) {
   out.n_cells = nc;
   out.jac.resize(nc);
   out.det.resize(nc);
+  out.affine.resize(nc);
+  out.vertices.resize(nc);
   for (int64_t c = 0; c < nc; ++c) {
     Vec x[8];
     for (int i = 0; i < 8; ++i) x[i] = node(cell_node(c, i));
+    for (int i = 0; i < 8; ++i)
+      for (int d = 0; d < 3; ++d) out.vertices[c][3 * i + d] = x[i][d];
     Vec e1 = Sub(x[1], x[0]), e2 = Sub(x[3], x[0]), e3 = Sub(x[4], x[0]);
     double h = std::max({ Norm(e1), Norm(e2), Norm(e3) });
     const Vec expect[8] = { x[0], x[1], Add(x[1], e2), x[3], x[4], Add(x[4], e1),
                             Add(Add(x[4], e1), e2), Add(x[4], e2) };
+    bool affine = true;
     for (int i = 0; i < 8; ++i)
-      if (Norm(Sub(expect[i], x[i])) > 1e-10 * h)
-        throw std::invalid_argument("non-affine hexahedron: only parallelepiped cells are supported");
+      if (Norm(Sub(expect[i], x[i])) > 1e-10 * h) affine = false;
+    out.affine[c] = affine ? 1 : 0;
     auto &j = out.jac[c];
-    for (int d = 0; d < 3; ++d) { j[d] = e1[d] / 2; j[3 + d] = e2[d] / 2; j[6 + d] = e3[d] / 2; }
-    out.det[c] = j[0] * (j[4] * j[8] - j[5] * j[7]) - j[1] * (j[3] * j[8] - j[5] * j[6])
-               + j[2] * (j[3] * j[7] - j[4] * j[6]);
-    if (!(out.det[c] > 0)) throw std::invalid_argument("inverted or degenerate hexahedron");
+    if (affine) {
+      for (int d = 0; d < 3; ++d) { j[d] = e1[d] / 2; j[3 + d] = e2[d] / 2; j[6 + d] = e3[d] / 2; }
+      out.det[c] = j[0] * (j[4] * j[8] - j[5] * j[7]) - j[1] * (j[3] * j[8] - j[5] * j[6])
+                 + j[2] * (j[3] * j[7] - j[4] * j[6]);
+      if (!(out.det[c] > 0)) throw std::invalid_argument("inverted or degenerate hexahedron");
+    } else {
+      // trilinear cell: the Jacobian varies; its centre value is kept here and
+      // the node and face-point values are evaluated by TrilinearJacobian
+      const double zero[3] = { 0, 0, 0 };
+      TrilinearJacobian(out.vertices[c].data(), zero, j.data());
+      out.det[c] = j[0] * (j[4] * j[8] - j[5] * j[7]) - j[1] * (j[3] * j[8] - j[5] * j[6])
+                 + j[2] * (j[3] * j[7] - j[4] * j[6]);
+      if (!(out.det[c] > 0)) throw std::invalid_argument("inverted or degenerate hexahedron");
+    }
   }
 
   // Face matching by sorted corner ids.
@@ -236,6 +282,10 @@ HexConnectivity BuildHexConnectivity(const mcfd_mesh_desc &mesh) {
     }
     Vec n = Unit(Cross(ra, rb)), t = Unit(ra);
     for (int d = 0; d < 3; ++d) { face.frame[d] = n[d]; face.frame[3 + d] = t[d]; }
+    for (int i = 0; i < 4; ++i) {
+      Vec x = node(cell_node(face.holder, kFaceCorners[fh][i]));
+      for (int d = 0; d < 3; ++d) face.corners[3 * i + d] = x[d];
+    }
     face.orient = 0;
     if (face.sharer >= 0) {
       int fs = face.sharer_face;

@@ -1,6 +1,6 @@
 // Host-side connectivity of conforming hexahedral meshes for the nodal FR
 // kernels: faces, holder/sharer, local face ids, sharer orientation, normal
-// frames and per-cell affine geometry.
+// frames and per-cell geometry (affine or trilinear).
 //
 // Semantics follow the reference's face construction:
 //   * interior faces: one per adjacency pair (holder = lower cell id), in the
@@ -42,8 +42,9 @@ struct HexFace {
   // face quadrature point (i, j) -> holder in-face (a, b), same encoding
   int8_t quad_to_holder;
   int32_t tag;             // boundary tag (interior: -1)
-  double frame[6];         // n, t (b = n x t)
+  double frame[6];         // n, t (b = n x t) at the face centre
   double shift[3];         // sharer coordinates = holder coordinates + shift
+  double corners[12];      // the holder's face corners in its CGNS face order (the face map)
 };
 
 struct HexConnectivity {
@@ -54,7 +55,20 @@ struct HexConnectivity {
   // J (rows d/dxi, d/deta, d/dzeta), det(J) and the centre.
   std::vector<std::array<double, 9>> jac;
   std::vector<double> det;
+  // per cell: the 8 CGNS corners and whether the map is affine (a
+  // parallelepiped); a trilinear cell keeps its centre values in jac / det
+  std::vector<std::array<double, 24>> vertices;
+  std::vector<uint8_t> affine;
 };
+
+// Trilinear (Hexahedron8) map of corners xv[8][3] at local point xi[3], and its
+// Jacobian j (rows d/dxi, d/deta, d/dzeta), coordinate/hexahedron.hpp:397-526.
+void TrilinearPoint(const double *xv, const double *xi, double *x);
+void TrilinearJacobian(const double *xv, const double *xi, double *j);
+// Normal frame {n, t} of the bilinear face map over corners xq[4][3] (CGNS
+// QUAD_4 order) at local (a, b): n = normalize(J_a x J_b), t = normalize(J_a)
+// (coordinate/face.hpp:54-62).
+void FaceFrameAt(const double *xq, double a, double b, double *frame);
 
 // Maps in-face indices (a, b) through an orientation code for q points.
 inline void ApplyOrient(int code, int q, int a, int b, int *a2, int *b2) {

@@ -323,7 +323,7 @@ int mcfd_startup_limiter(mcfd_solver *h, int32_t kind) {
           for (int k = 0; k < q; ++k) wl.push_back(w[i] * w[j] * w[k]);
       mcfd::DeviceArray<double> d_wl, d_det;
       d_wl.Upload(wl, h->stream);
-      d_det.Upload(g.conn.det, h->stream);
+      d_det.Upload(g.NodeDeterminants(), h->stream);
       Check(mcfd::LaunchFrAverage(g.u0.ptr, d_wl.ptr, d_det.ptr, g.n_cells, g.np, h->stream), "average launch");
       Check(hipStreamSynchronize(h->stream), "average sync");
       return;

@@ -79,8 +79,12 @@ struct Shape {
 // arithmetic and its order are those of the three-round form, so both give
 // bitwise the same results.  kPrefetch (with kOneRound): the neighbour lines
 // of that many face-task slots are loaded before the passes.
+//
+// kCurved (trilinear cells): J* and 1/det(J) are read per solution point, the
+// flux-point scale per cell flux point and the frame per face flux point (in
+// the holder's in-face order) instead of per cell and per face.
 template <int Q, int Cells, int Threads, int kRiemann, bool kDiag, bool kMixed, bool kOneRound = false,
-          int kPrefetch = 0>
+          int kPrefetch = 0, bool kCurved = false>
 __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTables &lt, int64_t block,
                                                double *lds) {
   using S = Shape<Q, Cells, Threads, kOneRound>;
@@ -115,9 +119,13 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
 #pragma unroll
       for (int k = 0; k < 5; ++k) u0v[k] = a.u0[base + tid * 5 + k];
     }
-    js = a.geo + (int64_t)a.geo_id[c0 + ncell] * kGeoStride;
+    if constexpr (kCurved) {
+      js = a.geo_node + ((c0 + ncell) * NP + node) * kNodeGeoStride;
+    } else {
+      js = a.geo + (int64_t)a.geo_id[c0 + ncell] * kGeoStride;
+      if (node < 6) s_scale[ncell * 6 + node] = js[10 + node];
+    }
     inv_det = js[9];
-    if (node < 6) s_scale[ncell * 6 + node] = js[10 + node];
 #pragma unroll
     for (int k = 0; k < 5; ++k) s_u[(ncell * 5 + k) * PAD + spos] = cu.v[k];
   }
@@ -307,14 +315,29 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
     }
     const int32_t nb = t_nb[t];
     const int32_t info = t_info[t];
-    const double *fr = a.frames + (int64_t)(info >> 7) * kFrameStride;
+    const double *fr;
+    double scale;
+    if constexpr (kCurved) {
+      // the face point in the holder's in-face order: this cell's own (a, b),
+      // or the neighbour's where the neighbour holds the face
+      int cp = ia * Q + ib;
+      if (!((info >> 6) & 1)) {
+        const int orient = (info >> 3) & 7;
+        const int s = (orient & 1) ? ib : ia, tt = (orient & 1) ? ia : ib;
+        cp = ((orient & 2) ? Q - 1 - s : s) * Q + ((orient & 4) ? Q - 1 - tt : tt);
+      }
+      fr = a.frames + ((int64_t)(info >> 7) * NFP + cp) * kFrameStride;
+      scale = a.fscale[((c0 + t_cell[t]) * 6 + f) * NFP + ia * Q + ib];
+    } else {
+      fr = a.frames + (int64_t)(info >> 7) * kFrameStride;
+      scale = s_scale[t_cell[t] * 6 + f];
+    }
     FaceFrame frame;
 #pragma unroll
     for (int d = 0; d < 3; ++d) { frame.n[d] = fr[d]; frame.t[d] = fr[3 + d]; }
     frame.b[0] = frame.n[1] * frame.t[2] - frame.n[2] * frame.t[1];
     frame.b[1] = frame.n[2] * frame.t[0] - frame.n[0] * frame.t[2];
     frame.b[2] = frame.n[0] * frame.t[1] - frame.n[1] * frame.t[0];
-    const double scale = s_scale[t_cell[t] * 6 + f];
     State fu;
     double sgn = 1.0;
     // the exact solver's state of this flux point, one copy per side: both
@@ -428,7 +451,8 @@ __device__ __forceinline__ void FrHexStageBody(const FrHexArgs &a, const LineTab
   }
 }
 
-template <int Q, int Cells, int Threads, int kRiemann, bool kDiag, bool kMixed, bool kOneRound, int kPrefetch>
+template <int Q, int Cells, int Threads, int kRiemann, bool kDiag, bool kMixed, bool kOneRound, int kPrefetch,
+          bool kCurved = false>
 __global__ __launch_bounds__(Threads) void FrHexStageKernel(FrHexArgs a, LineTables lt) {
   __shared__ double lds[Shape<Q, Cells, Threads, kOneRound>::kLdsDoubles];
   // The line tables are indexed by lane-varying node indices: read from the
@@ -440,7 +464,7 @@ __global__ __launch_bounds__(Threads) void FrHexStageKernel(FrHexArgs a, LineTab
     reinterpret_cast<double *>(&s_lt)[threadIdx.x] = reinterpret_cast<const double *>(&lt)[threadIdx.x];
   int64_t block = LogicalBlock(a.xcd_remap);
   if (a.tile_order) block = a.tile_order[block];
-  FrHexStageBody<Q, Cells, Threads, kRiemann, kDiag, kMixed, kOneRound, kPrefetch>(a, s_lt, block, lds);
+  FrHexStageBody<Q, Cells, Threads, kRiemann, kDiag, kMixed, kOneRound, kPrefetch, kCurved>(a, s_lt, block, lds);
 }
 
 // Single-mesh launch shape per Q: one thread per solution point.  Q = 4 and
@@ -559,6 +583,11 @@ hipError_t Launch(const FrHexArgs &args, const LineTables &lt, hipStream_t strea
   int64_t blocks = (args.n_cells + C - 1) / C;
   if (blocks > 0x7fffffff) return hipErrorInvalidValue;
   const dim3 grid((unsigned)blocks), block(W);
+  if (args.curved) {
+    if (args.mixed || args.diag_metric) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((FrHexStageKernel<Q, C, W, R, false, false, O, PF, true>), grid, block, 0, stream, args, lt);
+    return hipGetLastError();
+  }
   if (args.mixed) {
     if (args.diag_metric) {
       hipLaunchKernelGGL((FrHexStageKernel<Q, C, W, R, true, true, O, PF>), grid, block, 0, stream, args, lt);

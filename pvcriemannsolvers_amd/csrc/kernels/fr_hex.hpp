@@ -1,4 +1,4 @@
-// Launch interface of the nodal FR kernels on affine hexahedra.
+// Launch interface of the nodal FR kernels on hexahedra (affine or trilinear).
 #ifndef MINICFD_KERNELS_FR_HEX_HPP_
 #define MINICFD_KERNELS_FR_HEX_HPP_
 
@@ -28,6 +28,8 @@ struct LineTables {
 //   [16]    length = cbrt(volume) / 2
 constexpr int kGeoStride = 20;
 constexpr int kFrameStride = 6;  // n, t
+// Per-node record of a trilinear cell: J* (9, as above), 1 / det(J).
+constexpr int kNodeGeoStride = 10;
 
 struct FrHexArgs {
   const double *us;      // state of this stage, also read for neighbour traces
@@ -37,7 +39,11 @@ struct FrHexArgs {
   const int32_t *finfo;  // [cell][6]: frame id << 7 | holder << 6 | orient << 3 | neighbour face
   const int32_t *geo_id; // [cell]
   const double *geo;     // [geo][kGeoStride]
-  const double *frames;  // [frame][kFrameStride]
+  const double *frames;  // [frame][kFrameStride]; curved: [face][a * Q + b][kFrameStride]
+  // curved (trilinear) meshes: [cell][node][kNodeGeoStride] and the signed
+  // flux-point scales [cell][local face][a * Q + b]; null otherwise
+  const double *geo_node;
+  const double *fscale;
   const int32_t *bc_kind;   // [tag]: BcKind (euler.hpp)
   const double *bc_state;   // [tag][5] given state (conservative, or the subsonic parameters)
   // If set, the given state of every boundary flux point instead:
@@ -53,6 +59,7 @@ struct FrHexArgs {
   int32_t xcd_remap;     // remap block ids so each XCD sweeps a contiguous range
   int32_t diag_metric;   // every J* is diagonal (axis-aligned cells)
   int32_t mixed;         // FP32 flux-column contractions (mixed-precision form)
+  int32_t curved;        // per-node metrics and per-point scales and frames
   const int32_t *tile_order;  // [tile]: tile swept by logical block b (spatial locality), or null
   // If set, the time-step reduction of LaunchFrHexTimeStep over the state this
   // pass writes is folded into *dt_min_bits (same arithmetic, no extra pass).

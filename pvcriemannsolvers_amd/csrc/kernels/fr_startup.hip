@@ -23,7 +23,7 @@ __global__ __launch_bounds__(256) void FrAverageKernel(double *u, const double *
   double *uc = u + cell * np * 5;
   double sum = 0, volume = 0;
   for (int q = 0; q < np; ++q) {
-    const double w = MulRn(w_local[q], det[cell]);
+    const double w = MulRn(w_local[q], det[cell * np + q]);
     sum = AddRn(sum, MulRn(uc[q * 5 + k], w));
     volume = AddRn(volume, w);
   }

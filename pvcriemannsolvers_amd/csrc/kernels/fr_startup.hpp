@@ -10,7 +10,7 @@
 namespace mcfd {
 
 // u [cell][np][5] in place: every node takes the cell average with the
-// global weights w_local[q] * det[cell].
+// global weights w_local[q] * det[cell * np + q] (det(J) at each node).
 hipError_t LaunchFrAverage(double *u, const double *w_local, const double *det, int64_t n_cells, int np,
                            hipStream_t stream);
 // value [cell][5] = the majority value of each component; *failed |= 1 where

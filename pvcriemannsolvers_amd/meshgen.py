@@ -18,14 +18,26 @@ import numpy as np
 
 
 def box_hex(nx: int, ny: int, nz: int, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
-            periodic=(False, False, False), shuffle_seed: int | None = None) -> dict:
-    """Structured hexahedral box; cell order optionally shuffled by seed."""
+            periodic=(False, False, False), shuffle_seed: int | None = None, jitter: float = 0.0,
+            seed: int = 0) -> dict:
+    """Structured hexahedral box; cell order optionally shuffled by seed.
+
+    ``jitter`` moves every vertex off the box boundary by U(-jitter*h,
+    +jitter*h) per coordinate (h the spacing in that direction), seeded, so
+    the cells become general trilinear hexahedra with non-planar faces while
+    the box sides stay planar (and periodic partners stay translates).
+    """
     lo = np.asarray(lo, dtype=np.float64)
     hi = np.asarray(hi, dtype=np.float64)
     xs = [np.linspace(lo[d], hi[d], n + 1) for d, n in enumerate((nx, ny, nz))]
     # node id = (k * (ny + 1) + j) * (nx + 1) + i
     kk, jj, ii = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
     coords = np.stack([xs[0][ii], xs[1][jj], xs[2][kk]], axis=-1).reshape(-1, 3)
+    if jitter > 0:
+        rng = np.random.default_rng(seed)
+        h = (hi - lo) / np.array([nx, ny, nz])
+        inner = np.all((coords > lo + 0.5 * h) & (coords < hi - 0.5 * h), axis=1)
+        coords[inner] += rng.uniform(-jitter, jitter, (int(inner.sum()), 3)) * h
 
     def nid(i, j, k):
         return (k * (ny + 1) + j) * (nx + 1) + i

@@ -114,6 +114,9 @@ MESHES = {
     "rotated_cells_walls": (lambda: rotate_cells(meshgen.box_hex(4, 3, 3, shuffle_seed=9), 11), True),
     "rotated_cells_periodic": (lambda: rotate_cells(meshgen.box_hex(3, 3, 3, periodic=(True,) * 3), 12), False),
     "sheared_rotated": (lambda: sheared(rotate_cells(meshgen.box_hex(3, 2, 4, shuffle_seed=2), 13)), True),
+    "jittered_rotated": (lambda: rotate_cells(meshgen.box_hex(3, 3, 3, shuffle_seed=6, jitter=0.2, seed=10), 14),
+                         True),
+    "jittered_periodic": (lambda: meshgen.box_hex(3, 3, 3, periodic=(True,) * 3, jitter=0.2, seed=11), False),
 }
 
 
@@ -248,8 +251,9 @@ def test_unsupported_configurations_are_rejected():
         (dict(mesh=meshgen.box_tet(2, 2, 2), degree=3, scheme="dg", bcs=walls), C.MCFD_EINVAL, "degree"),
         # FR carries Q <= 6 points per direction
         (dict(mesh=meshgen.box_hex(2, 2, 2), degree=6, scheme="fr", bcs=walls), C.MCFD_EINVAL, "degree"),
-        # trilinear (non-parallelepiped) hexahedra
-        (dict(mesh=trilinear, degree=2, bcs=walls), C.MCFD_EINVAL, "parallelepiped"),
+        # trilinear (non-parallelepiped) hexahedra run with FR only
+        (dict(mesh=trilinear, degree=2, scheme="dg", bcs=walls), C.MCFD_ENOTSUP, "affine"),
+        (dict(mesh=trilinear, degree=2, bcs=walls, precision="mixed"), C.MCFD_ENOTSUP, "affine"),
     ]
     for kw, code, words in cases:
         mesh = kw.pop("mesh")

@@ -69,6 +69,9 @@ MESHES = {
     "sheared_rotated": (lambda: sheared(rotate_cells(meshgen.box_hex(2, 2, 1, shuffle_seed=3), 7)), BOX_BCS),
     "periodic_rotated": (lambda: rotate_cells(meshgen.box_hex(3, 2, 2, periodic=(True,) * 3, shuffle_seed=4), 8),
                          []),
+    # trilinear hexahedra with non-planar faces: per-node J*, per-point frames and scales
+    "jittered": (lambda: rotate_cells(meshgen.box_hex(3, 3, 2, shuffle_seed=5, jitter=0.2, seed=6), 9), BOX_BCS),
+    "jittered_periodic": (lambda: meshgen.box_hex(3, 3, 3, periodic=(True,) * 3, jitter=0.2, seed=7), []),
 }
 
 
