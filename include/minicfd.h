@@ -177,6 +177,16 @@ int mcfd_get_time_step(mcfd_solver *h, double cfl, double dt_guess, double *dt);
  * FR_GENERAL: solution points ([n_cells][N][3], polynomial/hexahedron.hpp:646-652).
  * DG_GENERAL: cell quadrature points ([n_cells][Nq][3], projection.hpp:28-43). */
 int mcfd_sample_points(const mcfd_solver *h, int32_t *n_per_cell, double *xyz);
+/* polynomial().LocalToGlobalAndValue of every cell (the evaluation that
+ * mesh/vtk.hpp:431-460 writes frames with): at n_local local points
+ * ([n_local][3]; hexahedra (xi, eta, zeta) in [-1, 1]^3 with the CGNS corners
+ * at +-1; tetrahedra with CGNS vertices 0..3 at (1,0,0), (0,1,0), (0,0,1),
+ * (0,0,0)) the global points xyz [n_cells][n_local][3] and the solution's
+ * values [n_cells][n_local][5].  `column` is a solution column in the layout
+ * of mcfd_get_solution_column, or null for the handle's current solution
+ * (MCFD_ESTATE on a host-only handle).  xyz or values may be null. */
+int mcfd_evaluate(mcfd_solver *h, const double *column, int32_t n_local, const double *local, double *xyz,
+                  double *values);
 /* Polynomial::Approximate (fem.hpp:147-154): build the solution from values
  * at the sample points ([n_cells][n_per_cell][5]) and make it current. */
 int mcfd_approximate(mcfd_solver *h, const double *values);

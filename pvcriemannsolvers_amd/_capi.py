@@ -69,6 +69,7 @@ SIGNATURES = {
     "mcfd_get_time_step": [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, _dp],
     "mcfd_sample_points": [ctypes.c_void_p, _i32p, _dp],
     "mcfd_approximate": [ctypes.c_void_p, _dp],
+    "mcfd_evaluate": [ctypes.c_void_p, _dp, ctypes.c_int32, _dp, _dp, _dp],
     "mcfd_face_count": [ctypes.c_void_p, _i64p, _i64p],
     "mcfd_faces": [ctypes.c_void_p, _i64p, _i64p, _i32p, _i32p, _i32p],
     "mcfd_limiter_flags": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint8), _i64p],

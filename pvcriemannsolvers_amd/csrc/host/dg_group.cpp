@@ -508,4 +508,34 @@ void DgGroup::Faces(int64_t *holder, int64_t *sharer, int32_t *holder_face, int3
   }
 }
 
+void DgGroup::Evaluate(const double *column, int n_local, const double *local, double *xyz,
+                       double *values) const {
+  const bool hex = cell_type == MCFD_HEXA_8;
+  const int packed = n * (n + 1) / 2;
+  for (int64_t c = 0; c < n_cells; ++c) {
+    const double *g = h_geo.data() + c * kDgGeoStride;  // J rows, |det J|, length, centre
+    const double *s = h_basis.data() + c * packed;
+    const double *uc = column + c * n * 5;
+    for (int p = 0; p < n_local; ++p) {
+      double eta[3];
+      for (int d = 0; d < 3; ++d) eta[d] = local[3 * p + d] - (hex ? 0.0 : 0.25);
+      double y[3];
+      for (int d = 0; d < 3; ++d) y[d] = eta[0] * g[d] + eta[1] * g[3 + d] + eta[2] * g[6 + d];
+      if (xyz)
+        for (int d = 0; d < 3; ++d) xyz[(c * n_local + p) * 3 + d] = g[11 + d] + y[d];
+      if (!values) continue;
+      // Taylor monomials (basis/taylor.hpp), then the orthonormal basis phi = S T
+      const double t[10] = { 1.0, y[0], y[1], y[2], y[0] * y[0], y[0] * y[1], y[0] * y[2],
+                             y[1] * y[1], y[1] * y[2], y[2] * y[2] };
+      double *v = values + (c * n_local + p) * 5;
+      for (int k = 0; k < 5; ++k) v[k] = 0;
+      for (int i = 0, e = 0; i < n; ++i) {
+        double phi = 0;
+        for (int jj = 0; jj <= i; ++jj, ++e) phi += s[e] * t[jj];
+        for (int k = 0; k < 5; ++k) v[k] += uc[i * 5 + k] * phi;
+      }
+    }
+  }
+}
+
 }  // namespace mcfd

@@ -53,6 +53,11 @@ struct DgGroup {
   void Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &scheme,
              const std::vector<int32_t> &bc_kinds, const std::vector<double> &bc_states);
   void Upload(hipStream_t stream);
+  // polynomial().LocalToGlobalAndValue of every cell at n_local local points
+  // (hexahedra: [-1, 1]^3; tetrahedra: the corners (1,0,0), (0,1,0), (0,0,1),
+  // (0,0,0) of CGNS vertices 0..3): xyz [cell][p][3], values [cell][p][5] of
+  // the modal column; either output may be null
+  void Evaluate(const double *column, int n_local, const double *local, double *xyz, double *values) const;
 
   size_t ColumnSize() const { return size_t(n_cells) * n * 5; }
   // Stage 0 residual, 1..3 SSP-RK3 stages (Taylor coefficients, then the stage kernel).

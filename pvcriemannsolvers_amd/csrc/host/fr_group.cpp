@@ -392,6 +392,43 @@ void FrHexGroup::SamplePoints(double *xyz) const {
   }
 }
 
+void FrHexGroup::Evaluate(const double *column, int n_local, const double *local, double *xyz,
+                          double *values) const {
+  std::vector<double> la(q), lb(q), lc(q);
+  for (int64_t c = 0; c < n_cells; ++c) {
+    const double *uc = column + c * np * 5;
+    for (int p = 0; p < n_local; ++p) {
+      const double *l = local + 3 * p;
+      if (xyz) {
+        double *x = xyz + (c * n_local + p) * 3;
+        if (curved) {
+          TrilinearPoint(conn.vertices[c].data(), l, x);
+        } else {
+          const auto &j = conn.jac[c];
+          const auto &x0 = corner0[c];
+          for (int d = 0; d < 3; ++d)
+            x[d] = x0[d] + (l[0] + 1) * j[d] + (l[1] + 1) * j[3 + d] + (l[2] + 1) * j[6 + d];
+        }
+      }
+      if (!values) continue;
+      for (int m = 0; m < q; ++m) {
+        la[m] = LagrangeValue(q, x1d.data(), m, l[0]);
+        lb[m] = LagrangeValue(q, x1d.data(), m, l[1]);
+        lc[m] = LagrangeValue(q, x1d.data(), m, l[2]);
+      }
+      double *v = values + (c * n_local + p) * 5;
+      for (int k = 0; k < 5; ++k) v[k] = 0;
+      for (int i = 0; i < q; ++i)
+        for (int jj = 0; jj < q; ++jj)
+          for (int kk = 0; kk < q; ++kk) {
+            const double w = la[i] * lb[jj] * lc[kk];
+            const double *u = uc + ((i * q + jj) * q + kk) * 5;
+            for (int k = 0; k < 5; ++k) v[k] += w * u[k];
+          }
+    }
+  }
+}
+
 std::vector<double> FrHexGroup::NodeDeterminants() const {
   std::vector<double> det(size_t(n_cells) * np);
   for (int64_t c = 0; c < n_cells; ++c)

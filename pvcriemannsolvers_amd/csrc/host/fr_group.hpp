@@ -59,6 +59,10 @@ struct FrHexGroup {
   // Kernel arguments of one pass: stage 0 residual, 1..3 SSP-RK3 stages.
   FrHexArgs Args(int stage, double dt) const;
   void SamplePoints(double *xyz) const;
+  // polynomial().LocalToGlobalAndValue of every cell at n_local local points
+  // (xi, eta, zeta in [-1, 1]^3): xyz [cell][p][3], values [cell][p][5] of
+  // the nodal column; either output may be null
+  void Evaluate(const double *column, int n_local, const double *local, double *xyz, double *values) const;
   // det(J) at every solution point, [cell][node] (the start-up average's weights)
   std::vector<double> NodeDeterminants() const;
   // Tag and flux points of every boundary face ([boundary face][a * Q + b][3],

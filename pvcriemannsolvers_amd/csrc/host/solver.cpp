@@ -248,6 +248,24 @@ int mcfd_sample_points(const mcfd_solver *h, int32_t *n_per_cell, double *xyz) {
   return MCFD_OK;
 }
 
+int mcfd_evaluate(mcfd_solver *h, const double *column, int32_t n_local, const double *local, double *xyz,
+                  double *values) {
+  if (!h || !local || n_local < 1) return Fail(MCFD_EINVAL, "bad argument");
+  if (values && !column && h->host_only) return Fail(MCFD_ESTATE, "handle was created host-only: pass a column");
+  return Guard([&] {
+    std::vector<double> own;
+    if (values && !column) {
+      own.resize(h->ColumnSize());
+      Check(hipMemcpyAsync(own.data(), h->Solution(), own.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           h->stream), "get solution");
+      Check(hipStreamSynchronize(h->stream), "get solution sync");
+      column = own.data();
+    }
+    if (h->dg) h->modal.Evaluate(column, n_local, local, xyz, values);
+    else h->group.Evaluate(column, n_local, local, xyz, values);
+  });
+}
+
 int mcfd_approximate(mcfd_solver *h, const double *values) {
   MCFD_REQUIRE_DEVICE(h);
   if (!values) return Fail(MCFD_EINVAL, "null values");

@@ -87,6 +87,7 @@ class FiniteElement(System):
         self.n_cells, self.n_per_cell, self.column_size = n_cells.value, n_per_cell.value, size.value
         self.degree = degree
         self.scheme = scheme
+        self.cell_type = md.cell_type
 
     def close(self):
         if getattr(self, "_h", None):
@@ -189,6 +190,18 @@ class FiniteElement(System):
         out = np.empty((self.n_cells, n.value, 3))
         C.check(C.load().mcfd_sample_points(self._h, ctypes.byref(n), C.ptr(out, C._dp)))
         return out
+
+    def evaluate(self, local, column=None):
+        """(xyz [n_cells, n, 3], values [n_cells, n, 5]) of every cell at the
+        local points `local` [n, 3] (polynomial().LocalToGlobalAndValue): the
+        current solution, or `column` (which a host-only handle needs)."""
+        local = np.ascontiguousarray(local, np.float64).reshape(-1, 3)
+        xyz = np.empty((self.n_cells, len(local), 3))
+        values = np.empty((self.n_cells, len(local), 5))
+        col = None if column is None else np.ascontiguousarray(column, np.float64).reshape(-1)
+        C.check(C.load().mcfd_evaluate(self._h, C.ptr(col, C._dp) if col is not None else None, len(local),
+                                       C.ptr(local, C._dp), C.ptr(xyz, C._dp), C.ptr(values, C._dp)))
+        return xyz, values
 
     def approximate(self, values):
         """Set the state from values at sample_points(): nodal values (FR) or
