@@ -216,6 +216,11 @@ int mcfd_faces(const mcfd_solver *h, int64_t *holder, int64_t *sharer,
 typedef void (*mcfd_bc_function)(int64_t n, const double *xyz, double t, double *states, void *user);
 int mcfd_boundary_points(const mcfd_solver *h, int32_t tag, int64_t *n_points, double *xyz);
 int mcfd_set_boundary_function(mcfd_solver *h, int32_t tag, mcfd_bc_function f, void *user);
+/* A boundary function of the point only (the f(x, t) of fem.hpp:196-222 that
+ * ignores t), given as its values at the tag's points ([n][5] in the order of
+ * mcfd_boundary_points): evaluated once instead of at every stage time.
+ * Replaces the tag's function; uploaded with the next residual. */
+int mcfd_set_boundary_values(mcfd_solver *h, int32_t tag, const double *states);
 
 /* limiter::Reconstruct(part, limiter) (limiter/reconstruct.hpp:22-54): one
  * pass of the handle's limiter over the current solution, as the reference's

@@ -77,6 +77,7 @@ SIGNATURES = {
     "mcfd_startup_limiter": [ctypes.c_void_p, ctypes.c_int32],
     "mcfd_boundary_points": [ctypes.c_void_p, ctypes.c_int32, _i64p, _dp],
     "mcfd_set_boundary_function": [ctypes.c_void_p, ctypes.c_int32, BC_FUNCTION, ctypes.c_void_p],
+    "mcfd_set_boundary_values": [ctypes.c_void_p, ctypes.c_int32, _dp],
     "mcfd_face_fluxes": [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, _dp, _dp, _dp,
                          ctypes.POINTER(ctypes.c_uint8), _dp],
     "mcfd_face_fluxes_noalloc": [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, _dp, _dp, _dp,

@@ -40,27 +40,51 @@ void BoundaryTable::SetFunction(int tag, mcfd_bc_function f, void *user) {
     if (face_tag[b] == tag) fn.faces.push_back((int64_t)b);
   fn.xyz = Points(tag);
   fn.values.assign(fn.faces.size() * nfp * 5, 0.0);
-  const size_t n = face_tag.size() * nfp * 5;
-  if (!host && n > 0) {
-    Check(hipHostMalloc((void **)&host, n * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
-    Check(hipEventCreateWithFlags(&copied, hipEventDisableTiming), "hipEventCreate");
-    Check(hipEventRecord(copied, 0), "hipEventRecord");
-    dev.Allocate(n);
-  }
-  if (host) {
-    // constant states everywhere; function tags are written at each new time
-    Check(hipEventSynchronize(copied), "event sync");
-    for (size_t b = 0; b < face_tag.size(); ++b)
-      for (int p = 0; p < nfp; ++p)
-        std::copy(const_state.begin() + face_tag[b] * 5, const_state.begin() + face_tag[b] * 5 + 5,
-                  host + (b * nfp + p) * 5);
-  }
+  EnsureStorage();
+  // the tag's constant state until the function writes it at a stage time
+  Check(hipEventSynchronize(copied), "event sync");
+  for (int64_t b : fn.faces)
+    for (int p = 0; p < nfp; ++p)
+      std::copy(const_state.begin() + tag * 5, const_state.begin() + tag * 5 + 5, host + (b * nfp + p) * 5);
+  dirty = true;
   have_time = false;
+}
+
+void BoundaryTable::SetValues(int tag, const double *states) {
+  if (tag < 0 || tag >= (int)fns.size()) throw std::invalid_argument("boundary tag out of range");
+  Function &fn = fns[tag];
+  fn.f = nullptr;
+  fn.user = nullptr;
+  EnsureStorage();
+  Check(hipEventSynchronize(copied), "event sync");
+  int64_t i = 0;
+  for (size_t b = 0; b < face_tag.size(); ++b)
+    if (face_tag[b] == tag)
+      for (int p = 0; p < nfp; ++p, ++i) std::copy(states + i * 5, states + i * 5 + 5, host + (b * nfp + p) * 5);
+  dirty = true;
+  have_time = false;
+}
+
+void BoundaryTable::EnsureStorage() {
+  const size_t n = face_tag.size() * nfp * 5;
+  if (host || n == 0) return;
+  Check(hipHostMalloc((void **)&host, n * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+  Check(hipEventCreateWithFlags(&copied, hipEventDisableTiming), "hipEventCreate");
+  Check(hipEventRecord(copied, 0), "hipEventRecord");
+  dev.Allocate(n);
+  // constant states everywhere; function and per-point tags overwrite theirs
+  for (size_t b = 0; b < face_tag.size(); ++b)
+    for (int p = 0; p < nfp; ++p)
+      std::copy(const_state.begin() + face_tag[b] * 5, const_state.begin() + face_tag[b] * 5 + 5,
+                host + (b * nfp + p) * 5);
 }
 
 const double *BoundaryTable::At(double t, hipStream_t stream) {
   if (!dev.ptr) return nullptr;
-  if (have_time && t == time) return dev.ptr;
+  bool timed = false;
+  for (auto &fn : fns) timed |= fn.f && !fn.faces.empty();
+  if (!timed && !dirty) return dev.ptr;                 // per-point values already on the device
+  if (timed && have_time && t == time && !dirty) return dev.ptr;
   Check(hipEventSynchronize(copied), "event sync");  // the previous upload has read `host`
   for (auto &fn : fns) {
     if (!fn.f || fn.faces.empty()) continue;
@@ -72,7 +96,8 @@ const double *BoundaryTable::At(double t, hipStream_t stream) {
   Check(hipMemcpyAsync(dev.ptr, host, face_tag.size() * nfp * 5 * sizeof(double), hipMemcpyHostToDevice, stream),
         "boundary values upload");
   Check(hipEventRecord(copied, stream), "hipEventRecord");
-  have_time = true;
+  dirty = false;
+  have_time = timed;
   time = t;
   return dev.ptr;
 }

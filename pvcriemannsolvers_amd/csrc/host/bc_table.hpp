@@ -36,6 +36,7 @@ struct BoundaryTable {
   hipEvent_t copied = nullptr;       // the last upload has read `host`
   bool have_time = false;
   double time = 0;
+  bool dirty = false;                // `host` holds values the device table lacks
 
   BoundaryTable() = default;
   BoundaryTable(const BoundaryTable &) = delete;
@@ -49,10 +50,19 @@ struct BoundaryTable {
   std::vector<double> Points(int tag) const;
   // Installs (or with f == null removes) the function of a tag.
   void SetFunction(int tag, mcfd_bc_function f, void *user);
-  // Device table to read at time t (uploaded on `stream` if t is new), or
-  // null while no tag has a function.
+  // Fixed values at every point of a tag ([n][5] in Points(tag) order): a
+  // boundary function of the point only, evaluated once; removes the tag's
+  // function.
+  void SetValues(int tag, const double *states);
+  // Device table to read at time t (tags with a function re-evaluated and the
+  // table uploaded on `stream` if t is new; per-point values uploaded once),
+  // or null while no tag has a function or per-point values.
   const double *At(double t, hipStream_t stream);
   bool Active() const { return dev.ptr != nullptr; }
+
+ private:
+  // Pinned host table, device table and event, filled with the tags' constants.
+  void EnsureStorage();
 };
 
 }  // namespace mcfd

@@ -313,6 +313,13 @@ int mcfd_set_boundary_function(mcfd_solver *h, int32_t tag, mcfd_bc_function f, 
   return Guard([&] { h->bct.SetFunction(tag, f, user); });
 }
 
+int mcfd_set_boundary_values(mcfd_solver *h, int32_t tag, const double *states) {
+  MCFD_REQUIRE_DEVICE(h);
+  if (tag < 0 || tag >= (int32_t)h->bct.fns.size()) return Fail(MCFD_EINVAL, "boundary tag out of range");
+  if (!states) return Fail(MCFD_EINVAL, "null states");
+  return Guard([&] { h->bct.SetValues(tag, states); });
+}
+
 int mcfd_limiter_reconstruct(mcfd_solver *h) {
   MCFD_REQUIRE_DEVICE(h);
   if (!h->dg || !h->modal.limiter) return Fail(MCFD_ESTATE, "the handle has no limiter");

@@ -655,38 +655,47 @@ __device__ double Smoothness(const double *c, int stride, const double *s, const
   return beta;
 }
 
-// weno.hpp:121-155 (Lazy::Borrow): one thread per (troubled cell, slot).
-// Slot i < n_adj projects neighbour i onto the cell's basis and shifts its
-// average onto the cell's; slot n_adj is the cell's own expansion.
+// weno.hpp:121-155 (Lazy::Borrow), then the smoothness of every slot.  Slot
+// i < n_adj projects neighbour i onto the cell's basis and shifts its average
+// onto the cell's; slot n_adj is the cell's own expansion.  A wavefront works
+// on CPW troubled cells at once, LPC = 64 / CPW lanes each: the per-point and
+// per-(slot, component) phases of a tetrahedron (14 points, 25 smoothness
+// tasks) would leave most of a 64-lane wavefront idle.  Every lane runs the
+// same number of loop trips and barriers; a lane group past the end of the
+// list only skips the work.
 template <bool kHex, int P>
 __global__ __launch_bounds__(64) void DgWenoProjectKernel(DgLimiterArgs a) {
   constexpr int N = P == 1 ? 4 : 10, PACKED = N * (N + 1) / 2, NQ = DgShape<kHex, P>::kNQ;
   constexpr int NA = kHex ? 6 : 4;          // neighbours at most
-  __shared__ double s_ct[NA][5 * N];        // neighbours' Taylor coefficients
-  __shared__ double s_shift[NA][3];         // own centre - neighbour centre
-  __shared__ double s_y[NQ][3];             // the cell's quadrature points, offsets from its centre
-  __shared__ double s_phi[NQ][N];           // its basis at them
-  __shared__ double s_w[NQ];                // and their global weights
-  __shared__ double s_f[NA][NQ][5];         // neighbour polynomials at them
-  __shared__ double s_pr[NA + 1][5 * N];    // projections, the cell's own last
-  const int lane = threadIdx.x;
+  constexpr int CPW = kHex ? 1 : 2;         // troubled cells per wavefront
+  constexpr int LPC = 64 / CPW;             // lanes per cell
+  __shared__ double s_ct[CPW][NA][5 * N];   // neighbours' Taylor coefficients
+  __shared__ double s_shift[CPW][NA][3];    // own centre - neighbour centre
+  __shared__ double s_y[CPW][NQ][3];        // the cell's quadrature points, offsets from its centre
+  __shared__ double s_phi[CPW][NQ][N];      // its basis at them
+  __shared__ double s_w[CPW][NQ];           // and their global weights
+  __shared__ double s_f[CPW][NA][NQ][5];    // neighbour polynomials at them
+  __shared__ double s_pr[CPW][NA + 1][5 * N];  // projections, the cell's own last
+  const int g = threadIdx.x / LPC, lane = threadIdx.x - (threadIdx.x / LPC) * LPC;
   const DgTables *tb = a.tables;
   const int count = *a.n_troubled;
-  for (int li = blockIdx.x; li < count; li += gridDim.x) {
-    const int64_t c = a.troubled[li];
-    const int na = a.n_adj[c];
+  for (int base = blockIdx.x * CPW; base < count; base += gridDim.x * CPW) {
+    const int li = base + g;
+    const bool on = li < count;
+    const int64_t c = on ? a.troubled[li] : 0;
+    const int na = on ? a.n_adj[c] : 0;
     const double *gc = a.geo + c * kDgGeoStride;
     const double *s = a.basis + c * PACKED;
-    for (int e = lane; e < na * 5 * N; e += 64) {
+    for (int e = lane; e < na * 5 * N; e += LPC) {
       const int sl = e / (5 * N), r = e - sl * (5 * N);
-      s_ct[sl][r] = a.ct[(int64_t)a.adj[c * 6 + sl] * 5 * N + r];
+      s_ct[g][sl][r] = a.ct[(int64_t)a.adj[c * 6 + sl] * 5 * N + r];
     }
     if (lane < na) {
       const double *go = a.geo + (int64_t)a.adj[c * 6 + lane] * kDgGeoStride;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) s_shift[lane][d] = gc[11 + d] - go[11 + d];
+      for (int d = 0; d < 3; ++d) s_shift[g][lane][d] = gc[11 + d] - go[11 + d];
     }
-    for (int q = lane; q < NQ; q += 64) {
+    for (int q = lane; on && q < NQ; q += LPC) {
       double y[3];
       Offset(gc, tb->vol_eta[q], y);
       double t[10];
@@ -699,47 +708,48 @@ __global__ __launch_bounds__(64) void DgWenoProjectKernel(DgLimiterArgs a) {
       for (int i = 0; i < N; ++i) {
         double phi = 0.0;
         for (int j = 0; j <= i; ++j) phi = __fma_rn(s[i * (i + 1) / 2 + j], t[j], phi);
-        s_phi[q][i] = phi;
+        s_phi[g][q][i] = phi;
       }
 #pragma unroll
-      for (int d = 0; d < 3; ++d) s_y[q][d] = y[d];
-      s_w[q] = tb->vol_w[q] * gc[9];
+      for (int d = 0; d < 3; ++d) s_y[g][q][d] = y[d];
+      s_w[g][q] = tb->vol_w[q] * gc[9];
     }
     __syncthreads();
-    for (int e = lane; e < na * NQ; e += 64) {
+    for (int e = lane; e < na * NQ; e += LPC) {
       const int sl = e / NQ, q = e - sl * NQ;
-      const double yo[3] = { s_y[q][0] + s_shift[sl][0], s_y[q][1] + s_shift[sl][1], s_y[q][2] + s_shift[sl][2] };
-      const State f = EvalTaylor<P>(s_ct[sl], yo);
+      const double yo[3] = { s_y[g][q][0] + s_shift[g][sl][0], s_y[g][q][1] + s_shift[g][sl][1],
+                             s_y[g][q][2] + s_shift[g][sl][2] };
+      const State f = EvalTaylor<P>(s_ct[g][sl], yo);
 #pragma unroll
-      for (int k = 0; k < 5; ++k) s_f[sl][q][k] = f.v[k];
+      for (int k = 0; k < 5; ++k) s_f[g][sl][q][k] = f.v[k];
     }
     __syncthreads();
     // projection.hpp:28-43 (Project) and ProjectionWrapper::operator+=(my
     // average - its average) on the constant coefficient
-    for (int e = lane; e < na * 5 * N; e += 64) {
+    for (int e = lane; e < na * 5 * N; e += LPC) {
       const int sl = e / (5 * N), r = e - sl * (5 * N), i = r / 5, k = r - (r / 5) * 5;
       double acc = 0.0;
-      for (int q = 0; q < NQ; ++q) acc += (s_f[sl][q][k] * s_phi[q][i]) * s_w[q];
+      for (int q = 0; q < NQ; ++q) acc += (s_f[g][sl][q][k] * s_phi[g][q][i]) * s_w[g][q];
       if (i == 0) {
         const double s00 = s[0];
         acc += (a.u[c * 5 * N + k] * s00 - acc * s00) / s00;
       }
-      s_pr[sl][r] = acc;
+      s_pr[g][sl][r] = acc;
       a.proj[(c * kLimiterSlots + sl) * 5 * N + r] = acc;
     }
-    for (int r = lane; r < 5 * N; r += 64) {
+    for (int r = lane; on && r < 5 * N; r += LPC) {
       const double v = a.u[c * 5 * N + r];
-      s_pr[na][r] = v;
+      s_pr[g][na][r] = v;
       a.proj[(c * kLimiterSlots + na) * 5 * N + r] = v;
     }
     if (a.mode < 3) {  // weno::Eigen measures smoothness in characteristic variables
       __syncthreads();
-      for (int e = lane; e < (na + 1) * 5; e += 64) {
+      for (int e = lane; on && e < (na + 1) * 5; e += LPC) {
         const int sl = e / 5, k = e - sl * 5;
-        a.beta[(c * kLimiterSlots + sl) * 5 + k] = Smoothness<kHex, P>(s_pr[sl] + k, 5, s, gc, tb);
+        a.beta[(c * kLimiterSlots + sl) * 5 + k] = Smoothness<kHex, P>(s_pr[g][sl] + k, 5, s, gc, tb);
       }
     }
-    __syncthreads();  // the next cell reuses the LDS buffers
+    __syncthreads();  // the next cells reuse the LDS buffers
   }
 }
 

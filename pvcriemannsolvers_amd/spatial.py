@@ -143,6 +143,15 @@ class FiniteElement(System):
         self._bc_callbacks[tag] = cb  # kept alive while the library may call it
         C.check(C.load().mcfd_set_boundary_function(self._h, int(tag), cb, None))
 
+    def set_boundary_values(self, tag, states):
+        """Fixed boundary values of a tag at its points (boundary_points(tag) order, [n, 5]):
+        a boundary function of the point only, evaluated once."""
+        states = np.ascontiguousarray(states, np.float64).reshape(-1, 5)
+        assert len(states) == len(self.boundary_points(tag))
+        C.check(C.load().mcfd_set_boundary_values(self._h, int(tag), C.ptr(states, C._dp)))
+        if hasattr(self, "_bc_callbacks"):
+            self._bc_callbacks.pop(tag, None)
+
     def limiter_reconstruct(self):
         """limiter::Reconstruct (limiter/reconstruct.hpp:22-54): one limiter pass over the current solution."""
         C.check(C.load().mcfd_limiter_reconstruct(self._h))

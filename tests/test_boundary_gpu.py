@@ -171,6 +171,41 @@ def test_time_dependent_smart_boundary(scheme):
     g.close()
 
 
+def frozen_left(xyz, t):
+    """A smart-boundary state that varies over the face only."""
+    return pulsing_left(xyz, 0.02)
+
+
+@pytest.mark.parametrize("scheme", ["fr", "dg"])
+def test_fixed_boundary_values_equal_a_point_function(scheme):
+    """mcfd_set_boundary_values (evaluated once) equals the same time-independent
+    boundary function evaluated at every stage time, bit for bit, and mixes with
+    a timed function on another tag."""
+    bcs = [("smart", LEFT), ("smart", RIGHT)] + [("inviscid_wall", None)] * 4
+    if scheme == "fr":
+        mesh = meshgen.box_hex(6, 2, 2, hi=(1.0, 0.2, 0.2), shuffle_seed=3)
+        make = lambda: spatial.FiniteElement(mesh, degree=2, riemann="hllc", bcs=bcs)  # noqa: E731
+    else:
+        mesh = meshgen.box_tet(6, 2, 1, hi=(1.0, 0.3, 0.15), jitter=0.1, seed=4)
+        make = lambda: spatial.FiniteElement(mesh, degree=2, scheme="dg", riemann="hllc", bcs=bcs,  # noqa: E731
+                                             limiter="weno")
+    a, b = make(), make()
+    for g in (a, b):
+        vals = np.where((g.sample_points()[..., 0] < 0.5)[..., None], LEFT, RIGHT)
+        g.approximate(vals)
+        g.set_boundary_function(1, lambda xyz, t: np.broadcast_to(RIGHT * (1 + 0.1 * np.sin(50 * t)), (len(xyz), 5)))
+    a.set_boundary_function(0, frozen_left)
+    b.set_boundary_values(0, frozen_left(b.boundary_points(0), 0.0))
+    t, dt = 0.0, 2e-3
+    for _ in range(4):
+        a.rk3_update(t, dt)
+        b.rk3_update(t, dt)
+        t += dt
+    assert np.array_equal(a.get_solution_column(), b.get_solution_column())
+    a.close()
+    b.close()
+
+
 # ---------------------------------------------------------------- start-up limiters
 @pytest.mark.parametrize("degree", [1, 2, 3])
 def test_startup_limiters_match_oracle(degree):
