@@ -12,6 +12,7 @@
 #ifndef MINI_SPATIAL_MI355X_HPP_
 #define MINI_SPATIAL_MI355X_HPP_
 
+#include <array>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -49,12 +50,14 @@ class FiniteElement : public temporal::System<double> {
     Check(mcfd_create(&mesh, &scheme, bcs.data(), (int32_t)bcs.size(), &h_));
     Check(mcfd_sizes(h_, &n_cells_, &n_per_cell_, &size_));
     degree_ = scheme.degree;
+    cell_type_ = mesh.cell_type;
   }
   FiniteElement(FiniteElement const &) = delete;
   FiniteElement &operator=(FiniteElement const &) = delete;
   FiniteElement(FiniteElement &&that) noexcept
       : h_(std::exchange(that.h_, nullptr)), n_cells_(that.n_cells_), n_per_cell_(that.n_per_cell_),
-        size_(that.size_), degree_(that.degree_) {}
+        size_(that.size_), degree_(that.degree_), cell_type_(that.cell_type_),
+        field_names_(std::move(that.field_names_)) {}
   ~FiniteElement() override {
     if (h_) mcfd_destroy(h_);
   }
@@ -135,6 +138,20 @@ class FiniteElement : public temporal::System<double> {
     return xyz;
   }
   void Approximate(std::vector<double> const &values) { Check(mcfd_approximate(h_, values.data())); }
+  // polynomial().LocalToGlobalAndValue of every cell at n local points
+  // (mesh/vtk.hpp:431-460): xyz [cell][n][3] and values [cell][n][5] of the
+  // current solution, or of `column` (which a host-only handle needs)
+  void Evaluate(std::vector<double> const &local, std::vector<double> *xyz, std::vector<double> *values,
+                double const *column = nullptr) const {
+    const int32_t n = (int32_t)(local.size() / 3);
+    xyz->resize((size_t)n_cells_ * n * 3);
+    values->resize((size_t)n_cells_ * n * 5);
+    Check(mcfd_evaluate(h_, column, n, local.data(), xyz->data(), values->data()));
+  }
+  // Part::SetFieldNames / GetFieldName (mesh/part.hpp:477-482), the names a frame gives the components
+  void SetFieldNames(std::array<std::string, 5> const &names) { field_names_ = names; }
+  std::string const &GetFieldName(int k) const { return field_names_.at(k); }
+  int32_t CellType() const { return cell_type_; }
   // RungeKutta<3>::Update (rk.hpp:31-49) fused on the device; the state stays in HBM
   void Rk3Update(double t_curr, double dt) { Check(mcfd_rk3_update(h_, t_curr, dt)); }
 
@@ -164,6 +181,10 @@ class FiniteElement : public temporal::System<double> {
   int32_t n_per_cell_ = 0;
   int64_t size_ = 0;
   int degree_ = 0;
+  int32_t cell_type_ = MCFD_HEXA_8;
+  // demo/euler/sourceless.cpp:109-110
+  std::array<std::string, 5> field_names_{ "Density", "MomentumX", "MomentumY", "MomentumZ",
+                                           "EnergyStagnationDensity" };
 };
 
 }  // namespace mi355x

@@ -111,6 +111,49 @@ def test_riemann_host_types_match_oracle(riemann):
     assert rel.max() < tol, np.unravel_index(rel.argmax(), rel.shape)
 
 
+@pytest.mark.parametrize("tet", [False, True], ids=["fr_trilinear_hex", "dg_tet"])
+def test_vtk_writer_cpp_equals_python_writer(tmp_path, tet):
+    """mini::mesh::vtk::Writer (C++) writes the frame that pvcriemannsolvers_amd.vtk
+    writes for the same host-only element and column."""
+    from pvcriemannsolvers_amd import meshgen, spatial, vtk
+    binary = os.path.join(CPP, "_build", "test_vtk_writer")
+    if not os.path.exists(binary):
+        subprocess.run(["make", "-C", CPP, os.path.join("_build", "test_vtk_writer")], check=True)
+    if tet:
+        mesh = meshgen.box_tet(2, 2, 1, jitter=0.1, seed=3)
+        fe = spatial.FiniteElement(mesh, degree=2, scheme="dg", riemann="hllc",
+                                   bcs=[("inviscid_wall", None)] * 6, host_only=True)
+    else:
+        mesh = meshgen.box_hex(3, 2, 2, jitter=0.2, seed=5)
+        fe = spatial.FiniteElement(mesh, degree=2, riemann="hllc", bcs=[("inviscid_wall", None)] * 6,
+                                   host_only=True)
+    column = np.random.default_rng(4).uniform(0.5, 2.0, fe.column_size)
+    data = os.path.join(str(tmp_path), "in.bin")
+    head = np.array([len(mesh["coords"]), len(mesh["cells"]), len(mesh["bface_tags"]), 2, int(tet)], np.int64)
+    with open(data, "wb") as f:
+        for a in (head, mesh["coords"].astype(np.float64), mesh["cells"].astype(np.int64),
+                  mesh["bface_nodes"].astype(np.int64), mesh["bface_tags"].astype(np.int64), column):
+            f.write(np.ascontiguousarray(a).tobytes())
+    r = subprocess.run([binary, data, os.path.join(str(tmp_path), "cpp")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "PASS" in r.stdout, r.stdout + r.stderr
+    pressure = ("Pressure", lambda xyz, v: 0.4 * (v[:, 4] - 0.5 * (v[:, 1:4] ** 2).sum(1) / v[:, 0]))
+    vtk.write_solutions(fe, os.path.join(str(tmp_path), "py"), "frame", column=column, point_data=[pressure],
+                        cell_data=[("CellId", lambda ids: ids.astype(float))])
+    got = vtk.read_piece(os.path.join(str(tmp_path), "cpp", "frame", "0.vtu"))
+    want = vtk.read_piece(os.path.join(str(tmp_path), "py", "frame", "0.vtu"))
+    assert np.array_equal(got["points"], want["points"])
+    for name in vtk.FIELD_NAMES:
+        assert np.array_equal(got["point_data"][name], want["point_data"][name]), name
+    p_got, p_want = got["point_data"]["Pressure"], want["point_data"]["Pressure"]
+    assert np.abs(p_got - p_want).max() <= 1e-15 * np.abs(p_want).max()
+    for key in ("connectivity", "offsets", "types"):
+        assert np.array_equal(got[key], want[key]), key
+    assert np.array_equal(got["cell_data"]["CellId"], want["cell_data"]["CellId"])
+    assert open(os.path.join(str(tmp_path), "cpp", "frame.pvtu")).read() == \
+        open(os.path.join(str(tmp_path), "py", "frame.pvtu")).read()
+    fe.close()
+
+
 @pytest.mark.gpu
 def test_system_contract_cpp_on_gpu():
     """FiniteElement driven by RungeKutta<3> through the System contract equals the fused device step bit for bit."""

@@ -77,6 +77,17 @@ def test_curved_bounded_residual(degree, riemann):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("degree", [2, 3])
+def test_curved_lumped_lobatto_residual(degree):
+    """fr::Lobatto (c_next = p / (2p + 1) on Lobatto nodes, i.e. DGSEM) on trilinear cells."""
+    o, g = make_pair(periodic_mesh(seed=9), degree, "roe", nodes="lobatto", c_next=degree / (2 * degree + 1))
+    u = density_wave(o.node_coords())
+    g.set_solution_column(u.reshape(-1))
+    assert rel(g.get_residual_column(), o.residual(u).reshape(-1)) < 1e-12
+    g.close()
+
+
+@pytest.mark.gpu
 def test_curved_free_stream():
     """A uniform flow stays uniform on trilinear cells (the analytic J* obeys
     the metric identities and the face normals are the J* columns' directions)."""
