@@ -6,6 +6,7 @@
 #include <cstring>
 #include <memory>
 #include <stdexcept>
+#include <thread>
 #include <vector>
 
 #include "core.hpp"
@@ -289,6 +290,39 @@ int oracle_fr_rk3(void *h, double *u, double dt, int n_steps) {
     size_t size = p->mesh.cells.size() * p->scheme->n * 5;
     for (int s = 0; s < n_steps; ++s)
       Rk3Step(size, u, dt, [&](double const *x, double *r) { p->scheme->Residual(x, r); });
+  });
+}
+// oracle_fr_rk3 with the residual and the RK combination spread over
+// `threads` host threads (bitwise the same states; CPU baseline timing).
+int oracle_fr_rk3_threaded(void *h, double *u, double dt, int n_steps, int threads) {
+  return Guard([&] {
+    auto *p = static_cast<FrHandle *>(h);
+    const size_t size = p->mesh.cells.size() * p->scheme->n * 5;
+    auto each = [threads, size](auto &&op) {  // elementwise over [0, size) on `threads` threads
+      std::vector<std::thread> pool;
+      for (int t = 0; t < threads; ++t)
+        pool.emplace_back([&, t] {
+          for (size_t i = size * t / threads; i < size * (t + 1) / threads; ++i) op(i);
+        });
+      for (auto &th : pool) th.join();
+    };
+    // Rk3Step with the same operation order per element; the buffers are
+    // written before they are read
+    auto buffer = [size] { return std::make_unique_for_overwrite<double[]>(size); };
+    auto u_curr = buffer(), r = buffer(), u1 = buffer(), u_next = buffer(), t = buffer();
+    auto next_solution = [&](double const *state, double *out) {
+      p->scheme->ResidualThreaded(state, r.get(), threads);
+      each([&](size_t i) { r[i] *= dt; out[i] = state[i] + r[i]; });
+    };
+    for (int s = 0; s < n_steps; ++s) {
+      each([&](size_t i) { u_curr[i] = u[i]; });
+      next_solution(u, u1.get());
+      each([&](size_t i) { u_next[i] = u_curr[i]; u_next[i] *= 3; });
+      next_solution(u1.get(), t.get());
+      each([&](size_t i) { u_next[i] += t[i]; u_next[i] /= 4; });
+      next_solution(u_next.get(), t.get());
+      each([&](size_t i) { t[i] *= 2; t[i] += u_curr[i]; t[i] /= 3; u[i] = t[i]; });
+    }
   });
 }
 double oracle_fr_time_step(void *h, double const *u, double cfl, double dt_guess) {

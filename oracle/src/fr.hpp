@@ -12,6 +12,10 @@
 #ifndef ORACLE_FR_HPP_
 #define ORACLE_FR_HPP_
 
+#include <algorithm>
+#include <memory>
+#include <stdexcept>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -272,6 +276,147 @@ struct FrScheme {
         for (int v = 0; v < 5; ++v) r[(c * n + ijk) * 5 + v] *= s;
       }
   }
+
+  // Residual with its loops spread over `threads` host threads, for the CPU
+  // baseline: cells split into ranges for the volume term, faces for the
+  // fluxes and gaps, then every cell gathers its faces' corrections in the
+  // order Residual applies them (interior faces by id; boundary faces by
+  // family, tag and id; points in order), so the result is bitwise that of
+  // Residual.
+  void ResidualThreaded(double const *u, double *r, int threads) {
+    if (mutation) throw std::runtime_error("planted mistakes run serially");
+    const size_t nc = mesh->cells.size();
+    const int nfp = q * q;
+    const size_t nf = mesh->faces.size();
+    auto parallel = [threads](size_t count, auto &&body) {
+      const int t_n = std::max(1, std::min<int>(threads, (int)count));
+      std::vector<std::thread> pool;
+      for (int t = 0; t < t_n; ++t)
+        pool.emplace_back([&, t] { body(count * t / t_n, count * (t + 1) / t_n); });
+      for (auto &th : pool) th.join();
+    };
+    if (cell_faces_.empty()) {  // each cell's (face, holder?) pairs in the order Residual visits them
+      cell_faces_.assign(nc, {});
+      for (int64_t f = 0; f < mesh->n_interior; ++f) {
+        cell_faces_[mesh->faces[f].holder].push_back({ f, true });
+        cell_faces_[mesh->faces[f].sharer].push_back({ f, false });
+      }
+      for (int family : kBcFamilyOrder)
+        for (size_t tag = 0; tag < bcs.size(); ++tag) {
+          if (bcs[tag].kind != family) continue;
+          for (size_t f = mesh->n_interior; f < nf; ++f)
+            if (mesh->faces[f].tag == (int)tag) cell_faces_[mesh->faces[f].holder].push_back({ (int64_t)f, true });
+        }
+    }
+    if (!flux_) {  // reused, every entry is written before it is read
+      flux_ = std::make_unique_for_overwrite<double[]>(nc * n * 15);
+      gaps_ = std::make_unique_for_overwrite<Val5[]>(nf * nfp * 2);
+    }
+    double *flux = flux_.get();
+    Val5 *gaps = gaps_.get();  // [face][point][holder, sharer]
+    parallel(nc, [&](size_t c0, size_t c1) {  // AddFluxDivergence, fr/general.hpp:271-288
+      std::fill(r + c0 * n * 5, r + c1 * n * 5, 0.0);
+      for (size_t c = c0; c < c1; ++c) {
+        for (int ijk = 0; ijk < n; ++ijk) {
+          Val5 uq;
+          for (int k = 0; k < 5; ++k) uq[k] = u[(c * n + ijk) * 5 + k];
+          Flux53 f = GetFluxMatrix(uq);
+          Mat3 const &js = jstar[c * n + ijk];
+          for (int k = 0; k < 5; ++k) for (int d = 0; d < 3; ++d)
+            flux[((c * n + ijk) * 5 + k) * 3 + d] = f[k][0] * js[0][d] + f[k][1] * js[1][d] + f[k][2] * js[2][d];
+        }
+        for (int i = 0; i < q; ++i) for (int j = 0; j < q; ++j) for (int k = 0; k < q; ++k) {
+          int ijk = (i * q + j) * q + k;
+          for (int v = 0; v < 5; ++v) {
+            double val = 0;
+            for (int m = 0; m < q; ++m) {
+              val += flux[((c * n + (m * q + j) * q + k) * 5 + v) * 3 + 0] * dmat[i][m];
+              val += flux[((c * n + (i * q + m) * q + k) * 5 + v) * 3 + 1] * dmat[j][m];
+              val += flux[((c * n + (i * q + j) * q + m) * 5 + v) * 3 + 2] * dmat[k][m];
+            }
+            r[(c * n + ijk) * 5 + v] -= val;
+          }
+        }
+      }
+    });
+    auto trace = [&](int64_t c, int i_face, int a, int b, Val5 *uf, Val5 *fn) {
+      int ijk[8];
+      FaceLine(q, i_face, a, b, ijk);
+      double const *l = lend[kFaceSign[i_face] > 0];
+      int dir = kFaceDir[i_face];
+      uf->fill(0); fn->fill(0);
+      for (int m = 0; m < q; ++m)
+        for (int v = 0; v < 5; ++v) {
+          (*uf)[v] += l[m] * u[(c * n + ijk[m]) * 5 + v];
+          (*fn)[v] += l[m] * flux[((c * n + ijk[m]) * 5 + v) * 3 + dir];
+        }
+    };
+    auto face_scale = [&](int64_t c, int i_face, int a, int b) {
+      Mat3 js = JstarAt(mesh->cells[c], i_face, a, b);
+      int dir = kFaceDir[i_face];
+      Vec3 col{ js[0][dir], js[1][dir], js[2][dir] };
+      return kFaceSign[i_face] * norm(col);
+    };
+    parallel(nf, [&](size_t f0, size_t f1) {
+      for (size_t f = f0; f < f1; ++f) {
+        auto const &face = mesh->faces[f];
+        for (int p = 0; p < nfp; ++p) {
+          auto const &ix = fp_index[f * nfp + p];
+          Val5 uh, fh;
+          trace(face.holder, face.holder_face, ix[0], ix[1], &uh, &fh);
+          Val5 fu;
+          if ((int64_t)f < mesh->n_interior) {
+            Val5 us, fs;
+            trace(face.sharer, face.sharer_face, ix[2], ix[3], &us, &fs);
+            fu = riemanns[f * nfp + p].GetFluxUpwind(face.frames[p], uh, us);
+            double ss = face_scale(face.sharer, face.sharer_face, ix[2], ix[3]);
+            for (int v = 0; v < 5; ++v) gaps[(f * nfp + p) * 2 + 1][v] = fu[v] * (-ss) - fs[v];
+          } else {
+            auto &rs = riemanns[f * nfp + p];
+            const int family = bcs[face.tag].kind;
+            Val5 const &given = point_state.empty() ? bcs[face.tag].state
+                                                    : point_state[(f - mesh->n_interior) * nfp + p];
+            switch (family) {
+            case kBcInviscidWall: fu = rs.GetFluxOnInviscidWall(face.frames[p], uh); break;
+            case kBcSupersonicOutlet: fu = rs.GetFluxOfState(face.frames[p], uh); break;
+            case kBcSupersonicInlet: fu = rs.GetFluxOfState(face.frames[p], given); break;
+            case kBcSubsonicInlet: fu = rs.GetFluxOnSubsonicInlet(face.frames[p], uh, given); break;
+            case kBcSubsonicOutlet: fu = rs.GetFluxOnSubsonicOutlet(face.frames[p], uh, given); break;
+            default: fu = rs.GetFluxOnSmartBoundary(face.frames[p], uh, given); break;
+            }
+          }
+          double sh = face_scale(face.holder, face.holder_face, ix[0], ix[1]);
+          for (int v = 0; v < 5; ++v) gaps[(f * nfp + p) * 2][v] = fu[v] * sh - fh[v];
+        }
+      }
+    });
+    parallel(nc, [&](size_t c0, size_t c1) {
+      for (size_t c = c0; c < c1; ++c) {
+        for (auto const &[f, holder] : cell_faces_[c]) {
+          auto const &face = mesh->faces[f];
+          const int i_face = holder ? face.holder_face : face.sharer_face;
+          double const *g = gprime[kFaceSign[i_face] > 0];
+          for (int p = 0; p < nfp; ++p) {
+            auto const &ix = fp_index[f * nfp + p];
+            const int a = holder ? ix[0] : ix[2], b = holder ? ix[1] : ix[3];
+            Val5 const &gap = gaps[(f * nfp + p) * 2 + (holder ? 0 : 1)];
+            int ijk[8];
+            FaceLine(q, i_face, a, b, ijk);
+            for (int m = 0; m < q; ++m)
+              for (int v = 0; v < 5; ++v) r[(c * n + ijk[m]) * 5 + v] -= gap[v] * g[m];
+          }
+        }
+        for (int ijk = 0; ijk < n; ++ijk) {  // divide by det(J) at each node, fr/general.hpp:249-268
+          double sc = 1.0 / det[c * n + ijk];
+          for (int v = 0; v < 5; ++v) r[(c * n + ijk) * 5 + v] *= sc;
+        }
+      }
+    });
+  }
+  // filled by ResidualThreaded
+  std::vector<std::vector<std::pair<int64_t, bool>>> cell_faces_;
+  std::unique_ptr<double[]> flux_;
+  std::unique_ptr<Val5[]> gaps_;
 
   // limiter/average.hpp:11-28 (kind 0): every node takes the cell average
   // sum_q u_q w_q / volume (polynomial/hexahedron.hpp:626-632);

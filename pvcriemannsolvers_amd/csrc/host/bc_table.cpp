@@ -41,6 +41,7 @@ void BoundaryTable::SetFunction(int tag, mcfd_bc_function f, void *user) {
   fn.xyz = Points(tag);
   fn.values.assign(fn.faces.size() * nfp * 5, 0.0);
   EnsureStorage();
+  if (!host) return;  // a mesh without boundary faces
   // the tag's constant state until the function writes it at a stage time
   Check(hipEventSynchronize(copied), "event sync");
   for (int64_t b : fn.faces)
@@ -56,6 +57,7 @@ void BoundaryTable::SetValues(int tag, const double *states) {
   fn.f = nullptr;
   fn.user = nullptr;
   EnsureStorage();
+  if (!host) return;  // a mesh without boundary faces
   Check(hipEventSynchronize(copied), "event sync");
   int64_t i = 0;
   for (size_t b = 0; b < face_tag.size(); ++b)

@@ -304,6 +304,14 @@ class FrOracle(_BoundaryValues):
         _check(load().oracle_fr_rk3(self.h, _ptr(u, _dp), float(dt), int(n_steps)))
         return u
 
+    def rk3_threaded(self, u, dt, n_steps=1, threads=4):
+        """rk3 with the residual spread over host threads (bitwise the same states)."""
+        lib = load()
+        lib.oracle_fr_rk3_threaded.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        u = f64(u).copy()
+        _check(lib.oracle_fr_rk3_threaded(self.h, _ptr(u, _dp), float(dt), int(n_steps), int(threads)))
+        return u
+
     def time_step(self, u, cfl, dt_guess=1e300):
         u = f64(u)
         return load().oracle_fr_time_step(self.h, _ptr(u, _dp), cfl, dt_guess)

@@ -206,6 +206,22 @@ def test_fixed_boundary_values_equal_a_point_function(scheme):
     b.close()
 
 
+def test_boundary_values_on_a_tag_without_faces():
+    """A tag that no boundary face carries (here: a periodic box) takes a
+    function or fixed values without effect."""
+    mesh = meshgen.box_hex(3, 3, 3, periodic=(True,) * 3)
+    g = spatial.FiniteElement(mesh, degree=1, riemann="hllc", bcs=[("smart", LEFT)])
+    g.set_solution_column(np.broadcast_to(LEFT, (g.n_cells, g.n_per_cell, 5)).reshape(-1))
+    before = g.get_residual_column()
+    assert len(g.boundary_points(0)) == 0
+    g.set_boundary_function(0, pulsing_left)
+    g.set_boundary_values(0, np.zeros((0, 5)))
+    g.rk3_update(0.0, 1e-3)
+    g.set_solution_column(np.broadcast_to(LEFT, (g.n_cells, g.n_per_cell, 5)).reshape(-1))
+    assert np.array_equal(g.get_residual_column(), before)
+    g.close()
+
+
 # ---------------------------------------------------------------- start-up limiters
 @pytest.mark.parametrize("degree", [1, 2, 3])
 def test_startup_limiters_match_oracle(degree):

@@ -178,3 +178,29 @@ def test_majority_limiter():
     o1 = ol.FrOracle(mesh, q=2, riemann=1, bcs=[(1, [0] * 5)] * 6)
     w = rng.random(4 * 8 * 5)
     np.testing.assert_array_equal(o1.startup_limiter(w, "majority"), w)
+
+
+@pytest.mark.parametrize("case", ["bounded_all_families", "periodic_jittered_exact"])
+def test_threaded_residual_is_bitwise_serial(case):
+    """The threaded oracle step (the CPU baseline's) equals the serial one bit
+    for bit: every node receives its face corrections in the serial order."""
+    if case == "bounded_all_families":
+        mesh = meshgen.box_hex(5, 3, 2, hi=(1.0, 0.3, 0.2), shuffle_seed=4)
+        state = cons(1.2, (60.0, 0.0, 0.0), 1e5)
+        # tags x-/x+/y-/y+/z-/z+: subsonic inlet, subsonic outlet, wall, smart, supersonic inlet, supersonic outlet
+        bcs = [(4, [1.2e5, 1.0, 0.0, 0.0, 300.0]), (5, [0, 0, 0, 0, 0.9e5]), (1, [0] * 5), (0, state),
+               (3, state), (2, [0] * 5)]
+        o = ol.FrOracle(mesh, q=3, riemann=1, bcs=bcs)
+        scale = np.array([1.2, 72.0, 1.0, 1.0, 2.5e5])
+    else:
+        mesh = meshgen.box_hex(3, 3, 3, periodic=(True,) * 3, jitter=0.2, seed=2)
+        o = ol.FrOracle(mesh, q=4, riemann=0, bcs=[])
+        scale = np.array([1.0, 0.3, 0.1, 0.05, 2.5])
+    x = o.node_coords()
+    wave = 1 + 0.05 * np.sin(2 * np.pi * x[..., 0]) * np.cos(2 * np.pi * x[..., 1])
+    u = (wave[..., None] * scale).reshape(-1)
+    dt = 1e-6 if case == "bounded_all_families" else 1e-3
+    serial = o.rk3(u, dt, n_steps=2)
+    assert np.isfinite(serial).all()
+    for threads in (1, 3, 8):
+        assert np.array_equal(o.rk3_threaded(u, dt, n_steps=2, threads=threads), serial), threads
