@@ -88,6 +88,48 @@ def test_curved_lumped_lobatto_residual(degree):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("degree,riemann", [(3, "exact"), (5, "hllc")])
+def test_curved_partial_tile_largest_degree(degree, riemann):
+    """Trilinear cells: 45 cells leave the last 2-cell workgroup (Q = 4) half
+    empty, Q = 6 runs one cell per workgroup; residual and RK3 steps against the
+    oracle.  (At Q = 6 the exact solver's stopping rule, f >= -1e-8, moves the
+    residual by about 1e-7 of its norm, so that degree runs with HLLC.)"""
+    mesh = meshgen.box_hex(5, 3, 3, periodic=(True,) * 3, jitter=0.2, seed=12, shuffle_seed=13)
+    o, g = make_pair(mesh, degree, riemann)
+    u = density_wave(o.node_coords(), seed=5)
+    g.set_solution_column(u.reshape(-1))
+    assert rel(g.get_residual_column(), o.residual(u).reshape(-1)) < TOL[riemann]
+    dt = o.time_step(u.reshape(-1), 0.1)
+    ref = o.rk3(u.reshape(-1), dt, n_steps=2)
+    for s in range(2):
+        g.rk3_update(s * dt, dt)
+    assert rel(g.get_solution_column(), ref) < TOL[riemann]
+    g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("degree", [1, 3])
+def test_single_trilinear_cell_with_every_boundary_family(degree):
+    """One trilinear cell (no interior face) whose six faces carry the six
+    boundary families, subsonic ones included."""
+    mesh = meshgen.box_hex(1, 1, 1, hi=(0.5, 0.4, 0.3))
+    mesh["coords"] = mesh["coords"].copy()
+    mesh["coords"][7] += (0.03, -0.02, 0.04)  # the corner (1, 1, 1): the cell is no longer a parallelepiped
+    state = conservative(1.2, (60.0, 5.0, -3.0), 1e5)
+    bcs = [("subsonic_inlet", [1.2e5, 1.0, 0.0, 0.0, 300.0]), ("subsonic_outlet", [0, 0, 0, 0, 0.9e5]),
+           ("inviscid_wall", None), ("smart", state), ("supersonic_inlet", state), ("supersonic_outlet", None)]
+    kinds = {"smart": 0, "inviscid_wall": 1, "supersonic_outlet": 2, "supersonic_inlet": 3, "subsonic_inlet": 4,
+             "subsonic_outlet": 5}
+    o = ol.FrOracle(mesh, q=degree + 1, riemann=1, bcs=[(kinds[k], s if s is not None else [0] * 5) for k, s in bcs])
+    g = spatial.FiniteElement(mesh, degree=degree, riemann="hllc", bcs=bcs)
+    x = o.node_coords()
+    u = conservative(1.2 * (1 + 0.02 * np.sin(7 * x[..., 0])), (60.0, 5.0, -3.0), 1e5 * (1 + 0.01 * x[..., 1]))
+    g.set_solution_column(u.reshape(-1))
+    assert rel(g.get_residual_column(), o.residual(u).reshape(-1)) < 1e-12
+    g.close()
+
+
+@pytest.mark.gpu
 def test_curved_free_stream():
     """A uniform flow stays uniform on trilinear cells (the analytic J* obeys
     the metric identities and the face normals are the J* columns' directions)."""
