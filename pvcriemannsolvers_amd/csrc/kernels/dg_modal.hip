@@ -754,43 +754,48 @@ __global__ __launch_bounds__(64) void DgWenoProjectKernel(DgLimiterArgs a) {
 }
 
 // weno.hpp:157-194 (Lazy::Reconstruct): nonlinear weights and the weighted
-// sum, one thread per (troubled cell, component); written over the state,
-// with its Taylor coefficients, only after every projection has been read.
+// sum, written over the state, with its Taylor coefficients, only after every
+// projection has been read.  A 256-thread workgroup takes 256 / (5 N)
+// troubled cells, one thread per (cell, coefficient, component), so the
+// projections, the state and the coefficients move as contiguous rows; each
+// thread forms the weights of its component itself (the same arithmetic for
+// all the component's threads), and the coefficients C S go through LDS.
 template <int P>
 __global__ __launch_bounds__(256) void DgWenoCombineKernel(DgLimiterArgs a) {
   constexpr int N = P == 1 ? 4 : 10, PACKED = N * (N + 1) / 2;
-  const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (w >= (int64_t)*a.n_troubled * 5) return;
-  const int64_t c = a.troubled[w / 5];
-  const int k = (int)(w - (w / 5) * 5);
-  const int na = a.n_adj[c];
-  double wt[kLimiterSlots];
-  for (int i = 0; i <= na; ++i) {
-    double wi = a.w0;
-    if (i == na) { wi *= -na; wi += 1.0; }
-    double b = a.beta[(c * kLimiterSlots + i) * 5 + k];
-    b += a.eps;
-    b *= b;
-    wt[i] = wi / b;
+  constexpr int ROW = 5 * N, CPB = 256 / ROW;  // cells per workgroup
+  __shared__ double s_fresh[CPB][ROW];
+  const int cs = threadIdx.x / ROW, r = threadIdx.x - (threadIdx.x / ROW) * ROW;
+  const int i = r / 5, k = r - (r / 5) * 5;
+  const int64_t li = (int64_t)blockIdx.x * CPB + cs;
+  const bool on = cs < CPB && li < *a.n_troubled;
+  const int64_t c = on ? a.troubled[li] : 0;
+  if (on) {
+    const int na = a.n_adj[c];
+    double wt[kLimiterSlots];
+    for (int sl = 0; sl <= na; ++sl) {
+      double wi = a.w0;
+      if (sl == na) { wi *= -na; wi += 1.0; }
+      double bb = a.beta[(c * kLimiterSlots + sl) * 5 + k];
+      bb += a.eps;
+      bb *= bb;
+      wt[sl] = wi / bb;
+    }
+    double sum = 0.0;
+    for (int sl = 0; sl <= na; ++sl) sum += wt[sl];
+    const double *pr = a.proj + c * kLimiterSlots * ROW;
+    double v = pr[na * ROW + r] * (wt[na] / sum);
+    for (int sl = 0; sl < na; ++sl) v += pr[sl * ROW + r] * (wt[sl] / sum);
+    a.u[c * ROW + r] = v;
+    s_fresh[cs][r] = v;
   }
-  double sum = 0.0;
-  for (int i = 0; i <= na; ++i) sum += wt[i];
-  const double *pr = a.proj + c * kLimiterSlots * 5 * N;
-  double fresh[10];
-#pragma unroll
-  for (int i = 0; i < N; ++i) {
-    double v = pr[na * 5 * N + 5 * i + k] * (wt[na] / sum);
-    for (int s = 0; s < na; ++s) v += pr[s * 5 * N + 5 * i + k] * (wt[s] / sum);
-    fresh[i] = v;
-    a.u[c * 5 * N + 5 * i + k] = v;
-  }
-  const double *sc = a.basis + c * PACKED;
-#pragma unroll
-  for (int j = 0; j < N; ++j) {
+  __syncthreads();
+  if (on) {
+    const double *sc = a.basis + c * PACKED;
+    const int j = i;  // this thread's Taylor coefficient
     double acc = 0.0;
-#pragma unroll
-    for (int i = j; i < N; ++i) acc = __fma_rn(fresh[i], sc[i * (i + 1) / 2 + j], acc);
-    a.ct[c * 5 * N + 5 * j + k] = acc;
+    for (int m = j; m < N; ++m) acc = __fma_rn(s_fresh[cs][5 * m + k], sc[m * (m + 1) / 2 + j], acc);
+    a.ct[c * ROW + r] = acc;
   }
 }
 
@@ -1011,8 +1016,8 @@ hipError_t LaunchDgLimiter(int cell_type, int degree, const DgLimiterArgs &a, hi
   if (b_flags > 0x7fffffff) return hipErrorInvalidValue;
   // one 64-thread workgroup per troubled cell at a time, striding over the list
   const int64_t b_proj = std::min<int64_t>(a.n_cells, 8192);
-  const int64_t b_comb = (a.n_cells * 5 + 255) / 256;
-  (void)n;
+  const int64_t b_comb = (a.n_cells * 5 + 255) / 256;  // Eigen combine: one thread per (cell, component)
+  const int64_t b_lazy = (a.n_cells + 256 / (5 * n) - 1) / (256 / (5 * n));  // Lazy combine: 256 / (5 N) cells
   if (b_comb > 0x7fffffff) return hipErrorInvalidValue;
   const int64_t b_eig = (a.n_cells * 30 + 255) / 256;
   if (b_eig > 0x7fffffff) return hipErrorInvalidValue;
@@ -1027,7 +1032,7 @@ hipError_t LaunchDgLimiter(int cell_type, int degree, const DgLimiterArgs &a, hi
     hipLaunchKernelGGL((DgWenoProjectKernel<false, P>), dim3((unsigned)b_proj), dim3(64), 0, stream, a); \
   }                                                                                                     \
   if (!eigen) {                                                                                         \
-    hipLaunchKernelGGL(DgWenoCombineKernel<P>, dim3((unsigned)b_comb), dim3(256), 0, stream, a);        \
+    hipLaunchKernelGGL(DgWenoCombineKernel<P>, dim3((unsigned)b_lazy), dim3(256), 0, stream, a);        \
   } else {                                                                                              \
     if (hex) {                                                                                          \
       hipLaunchKernelGGL((DgWenoEigenKernel<true, P>), dim3((unsigned)b_eig), dim3(256), 0, stream, a); \
