@@ -537,7 +537,7 @@ __global__ void DgLimiterResetKernel(int32_t *n_troubled) { *n_troubled = 0; }
 // centre, and the group's first lane sums the jumps in neighbour order (the
 // reference's order, so the flags are bitwise the same) and takes the
 // maxima.  Every flagged cell is also appended to the troubled-cell list (one
-// atomic per wavefront; the list order is irrelevant, each cell's
+// atomic per workgroup; the list order is irrelevant, each cell's
 // reconstruction only depends on the cell).
 constexpr int FlagLanes(bool hex) { return hex ? 8 : 4; }  // lanes per cell, a power of two dividing 64
 
@@ -598,12 +598,21 @@ __global__ __launch_bounds__(256) void DgWenoFlagsKernel(DgLimiterArgs a) {
     }
     a.flags[c] = rough ? 1 : 0;
   }
+  // append the flagged cells: one atomic per workgroup (flags cluster in
+  // space, and late in a run most wavefronts hold some)
+  __shared__ int s_count[4];
+  __shared__ int s_base;
   const unsigned long long ballot = __ballot(rough);
-  if (ballot == 0) return;
-  const int leader = __ffsll((long long)ballot) - 1;
-  int base = 0;
-  if (lane == leader) base = atomicAdd(a.n_troubled, __popcll(ballot));
-  base = __shfl(base, leader);
+  const int wave = (int)(threadIdx.x >> 6);
+  if (lane == 0) s_count[wave] = (int)__popcll(ballot);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int total = s_count[0] + s_count[1] + s_count[2] + s_count[3];
+    s_base = total > 0 ? atomicAdd(a.n_troubled, total) : 0;
+  }
+  __syncthreads();
+  int base = s_base;
+  for (int v = 0; v < wave; ++v) base += s_count[v];
   if (rough) a.troubled[base + __popcll(ballot & ((1ull << lane) - 1))] = (int32_t)c;
 }
 
