@@ -618,8 +618,12 @@ __global__ __launch_bounds__(256) void DgWenoFlagsKernel(DgLimiterArgs a) {
 
 // weno.hpp:74-95 (GetSmoothness) of the scalar expansion c[stride * i]
 // (i < N) on the basis `s` of a cell with geometry `gc`.
+// `ys`, `ws` (optional): the cell's quadrature points as offsets from its
+// centre and their global weights, already formed (the same arithmetic as
+// Offset and vol_w * |det J| below).
 template <bool kHex, int P>
-__device__ double Smoothness(const double *c, int stride, const double *s, const double *gc, const DgTables *tb) {
+__device__ double Smoothness(const double *c, int stride, const double *s, const double *gc, const DgTables *tb,
+                             const double (*ys)[3] = nullptr, const double *ws = nullptr) {
   constexpr int N = P == 1 ? 4 : 10, NQ = DgShape<kHex, P>::kNQ;
   double tc[10];  // Taylor coefficients (c S)[j]
 #pragma unroll
@@ -633,8 +637,12 @@ __device__ double Smoothness(const double *c, int stride, const double *s, const
   for (int j = 0; j < N; ++j) integ[j] = 0.0;
   for (int q = 0; q < NQ; ++q) {
     double y[3];
-    Offset(gc, tb->vol_eta[q], y);
-    const double w = tb->vol_w[q] * gc[9];
+    if (ys) {
+      y[0] = ys[q][0]; y[1] = ys[q][1]; y[2] = ys[q][2];
+    } else {
+      Offset(gc, tb->vol_eta[q], y);
+    }
+    const double w = ws ? ws[q] : tb->vol_w[q] * gc[9];
     double pdv[10];  // taylor.hpp:241-281 (GetPartialDerivatives)
     pdv[0] = 0.0;
 #pragma unroll
@@ -685,6 +693,7 @@ __global__ __launch_bounds__(64) void DgWenoProjectKernel(DgLimiterArgs a) {
   __shared__ double s_w[CPW][NQ];           // and their global weights
   __shared__ double s_f[CPW][NA][NQ][5];    // neighbour polynomials at them
   __shared__ double s_pr[CPW][NA + 1][5 * N];  // projections, the cell's own last
+  __shared__ double s_basis[CPW][PACKED];   // the cell's S
   const int g = threadIdx.x / LPC, lane = threadIdx.x - (threadIdx.x / LPC) * LPC;
   const DgTables *tb = a.tables;
   const int count = *a.n_troubled;
@@ -694,11 +703,13 @@ __global__ __launch_bounds__(64) void DgWenoProjectKernel(DgLimiterArgs a) {
     const int64_t c = on ? a.troubled[li] : 0;
     const int na = on ? a.n_adj[c] : 0;
     const double *gc = a.geo + c * kDgGeoStride;
-    const double *s = a.basis + c * PACKED;
+    const double *s = s_basis[g];
     for (int e = lane; e < na * 5 * N; e += LPC) {
       const int sl = e / (5 * N), r = e - sl * (5 * N);
       s_ct[g][sl][r] = a.ct[(int64_t)a.adj[c * 6 + sl] * 5 * N + r];
     }
+    for (int e = lane; on && e < PACKED; e += LPC) s_basis[g][e] = a.basis[c * PACKED + e];
+    __syncthreads();
     if (lane < na) {
       const double *go = a.geo + (int64_t)a.adj[c * 6 + lane] * kDgGeoStride;
 #pragma unroll
@@ -755,7 +766,7 @@ __global__ __launch_bounds__(64) void DgWenoProjectKernel(DgLimiterArgs a) {
       __syncthreads();
       for (int e = lane; on && e < (na + 1) * 5; e += LPC) {
         const int sl = e / 5, k = e - sl * 5;
-        a.beta[(c * kLimiterSlots + sl) * 5 + k] = Smoothness<kHex, P>(s_pr[g][sl] + k, 5, s, gc, tb);
+        a.beta[(c * kLimiterSlots + sl) * 5 + k] = Smoothness<kHex, P>(s_pr[g][sl] + k, 5, s, gc, tb, s_y[g], s_w[g]);
       }
     }
     __syncthreads();  // the next cells reuse the LDS buffers
