@@ -411,10 +411,15 @@ void DgGroup::Upload(hipStream_t stream) {
     nadj.Upload(h_nadj, stream);
     flags.Allocate(n_cells);
     Check(hipMemsetAsync(flags.ptr, 0, n_cells, stream), "memset");
-    std::vector<double> h_rec(size_t(n_cells) * kDgFlagRecord, 0.0);  // centre and volume; the rest per state
+    std::vector<double> h_rec(size_t(n_cells) * kDgFlagRecord, 0.0);  // geometry terms; the rest per state
     for (int64_t c = 0; c < n_cells; ++c) {
       for (int d = 0; d < 3; ++d) h_rec[c * kDgFlagRecord + kDgFlagCentre + d] = h_geo[c * kDgGeoStride + 11 + d];
       h_rec[c * kDgFlagRecord + kDgFlagVolume] = h_geo[c * kDgGeoStride + 14];
+      h_rec[c * kDgFlagRecord + kDgFlagDivisor] = std::pow(h_geo[c * kDgGeoStride + 14], (degree + 1.0) / 2.0 / 3);
+      const double s00 = h_basis[c * (n * (n + 1) / 2)];
+      const double measure = 1 / (s00 * s00);  // OrthoNormal::Measure
+      h_rec[c * kDgFlagRecord + kDgFlagScale1] = std::pow(measure, 2. / 3 - 1);
+      h_rec[c * kDgFlagRecord + kDgFlagScale2] = std::pow(measure, 4. / 3 - 1);
     }
     flag_rec.Upload(h_rec, stream);
     troubled.Allocate(size_t(n_cells) + 1);

@@ -591,7 +591,7 @@ __global__ __launch_bounds__(256) void DgWenoFlagsKernel(DgLimiterArgs a) {
         max_abs0 = fmax(max_abs0, max0);
         max_abs4 = fmax(max_abs4, max4);
       }
-      double divisor = pow(rc[kDgFlagVolume], (P + 1.0) / 2.0 / 3);
+      double divisor = rc[kDgFlagDivisor];  // pow(volume, (P + 1) / 6), set on the host
       divisor *= na;
       const double reference = P < 3 ? 1.0 : 3.0;
       rough = sum0 / max_abs0 / divisor > reference || sum4 / max_abs4 / divisor > reference;
@@ -617,13 +617,15 @@ __global__ __launch_bounds__(256) void DgWenoFlagsKernel(DgLimiterArgs a) {
 }
 
 // weno.hpp:74-95 (GetSmoothness) of the scalar expansion c[stride * i]
-// (i < N) on the basis `s` of a cell with geometry `gc`.
+// (i < N) on the basis `s` of a cell with geometry `gc`.  `vpow` holds the
+// cell's volume^(2 o / 3 - 1) for derivative orders o = 1, 2, with the volume
+// of OrthoNormal::Measure, 1 / S00^2 (the flag record's scale slots).
 // `ys`, `ws` (optional): the cell's quadrature points as offsets from its
 // centre and their global weights, already formed (the same arithmetic as
 // Offset and vol_w * |det J| below).
 template <bool kHex, int P>
 __device__ double Smoothness(const double *c, int stride, const double *s, const double *gc, const DgTables *tb,
-                             const double (*ys)[3] = nullptr, const double *ws = nullptr) {
+                             const double *vpow, const double (*ys)[3] = nullptr, const double *ws = nullptr) {
   constexpr int N = P == 1 ? 4 : 10, NQ = DgShape<kHex, P>::kNQ;
   double tc[10];  // Taylor coefficients (c S)[j]
 #pragma unroll
@@ -656,17 +658,15 @@ __device__ double Smoothness(const double *c, int stride, const double *s, const
 #pragma unroll
     for (int j = 0; j < N; ++j) integ[j] += pdv[j] * pdv[j] * w;
   }
-  const double s00 = s[0];
-  const double volume = 1 / (s00 * s00);  // OrthoNormal::Measure
   double beta = integ[1];
   beta += integ[2];
   beta += integ[3];
-  beta *= pow(volume, 2.0 / 3 - 1);
+  beta *= vpow[0];
   if constexpr (P >= 2) {
     double b2 = integ[4];
 #pragma unroll
     for (int j = 5; j < 10; ++j) b2 += integ[j];
-    b2 *= pow(volume, 4.0 / 3 - 1) / 4.0;
+    b2 *= vpow[1] / 4.0;
     beta += b2;
   }
   return beta;
@@ -694,6 +694,7 @@ __global__ __launch_bounds__(64) void DgWenoProjectKernel(DgLimiterArgs a) {
   __shared__ double s_f[CPW][NA][NQ][5];    // neighbour polynomials at them
   __shared__ double s_pr[CPW][NA + 1][5 * N];  // projections, the cell's own last
   __shared__ double s_basis[CPW][PACKED];   // the cell's S
+  __shared__ double s_vpow[CPW][2];         // its smoothness volume scales
   const int g = threadIdx.x / LPC, lane = threadIdx.x - (threadIdx.x / LPC) * LPC;
   const DgTables *tb = a.tables;
   const int count = *a.n_troubled;
@@ -709,6 +710,7 @@ __global__ __launch_bounds__(64) void DgWenoProjectKernel(DgLimiterArgs a) {
       s_ct[g][sl][r] = a.ct[(int64_t)a.adj[c * 6 + sl] * 5 * N + r];
     }
     for (int e = lane; on && e < PACKED; e += LPC) s_basis[g][e] = a.basis[c * PACKED + e];
+    if (on && lane < 2) s_vpow[g][lane] = a.flag_rec[c * kDgFlagRecord + kDgFlagScale1 + lane];
     __syncthreads();
     if (lane < na) {
       const double *go = a.geo + (int64_t)a.adj[c * 6 + lane] * kDgGeoStride;
@@ -766,7 +768,8 @@ __global__ __launch_bounds__(64) void DgWenoProjectKernel(DgLimiterArgs a) {
       __syncthreads();
       for (int e = lane; on && e < (na + 1) * 5; e += LPC) {
         const int sl = e / 5, k = e - sl * 5;
-        a.beta[(c * kLimiterSlots + sl) * 5 + k] = Smoothness<kHex, P>(s_pr[g][sl] + k, 5, s, gc, tb, s_y[g], s_w[g]);
+        a.beta[(c * kLimiterSlots + sl) * 5 + k] = Smoothness<kHex, P>(s_pr[g][sl] + k, 5, s, gc, tb, s_vpow[g], s_y[g],
+                                                                    s_w[g]);
       }
     }
     __syncthreads();  // the next cells reuse the LDS buffers
@@ -897,7 +900,7 @@ __global__ __launch_bounds__(256) void DgWenoEigenKernel(DgLimiterArgs a) {
       for (int m = 0; m < 5; ++m) v += l[k][m] * pr[i * 5 * N + 5 * j + m];
       ch[j] = v;
     }
-    double beta = Smoothness<kHex, P>(ch, 1, s, gc, a.tables);
+    double beta = Smoothness<kHex, P>(ch, 1, s, gc, a.tables, a.flag_rec + c * kDgFlagRecord + kDgFlagScale1);
     double wi = a.w0;
     if (i == na) { wi *= -na; wi += 1.0; }
     beta += a.eps;

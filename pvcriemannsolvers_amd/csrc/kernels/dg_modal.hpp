@@ -32,6 +32,10 @@ constexpr int kDgFaceStride = 8;        // n (3), t (3), area factor (1), pad
 // stage pass rewrite the rest for every state they produce.
 constexpr int kDgFlagRecord = 32;
 constexpr int kDgFlagCentre = 20, kDgFlagVolume = 23, kDgFlagAverage = 24;
+// Set once on the host with the C library's pow: the detector's volume power
+// volume^((p + 1) / 6) (weno.hpp:252) and the smoothness scales
+// (1 / S00^2)^(2/3 - 1), (1 / S00^2)^(4/3 - 1) (weno.hpp:74-95).
+constexpr int kDgFlagDivisor = 26, kDgFlagScale1 = 27, kDgFlagScale2 = 28;
 
 // Reference-element tables of one (cell type, degree); lives in device memory.
 struct DgTables {
