@@ -424,7 +424,6 @@ void DgGroup::Upload(hipStream_t stream) {
     flag_rec.Upload(h_rec, stream);
     troubled.Allocate(size_t(n_cells) + 1);
     proj.Allocate(size_t(n_cells) * 7 * 5 * n);
-    beta.Allocate(size_t(n_cells) * 7 * 5);
     if (limiter >= MCFD_LIMITER_WENO_EIGEN) eig.Allocate(size_t(n_cells) * 6 * 5 * n);
   }
 }
@@ -435,7 +434,7 @@ void DgGroup::Limit(double *u, double *ct, hipStream_t stream) {
   a.u = u; a.ct = ct; a.basis = basis.ptr; a.geo = geo.ptr;
   a.adj = adj.ptr; a.adj_face = adj_face.ptr; a.n_adj = nadj.ptr; a.faces = face_rec.ptr; a.tables = tables_d.ptr;
   a.flag_rec = flag_rec.ptr; a.flags = flags.ptr; a.troubled = troubled.ptr; a.n_troubled = troubled.ptr + n_cells;
-  a.proj = proj.ptr; a.beta = beta.ptr; a.eig = eig.ptr;
+  a.proj = proj.ptr; a.eig = eig.ptr;
   a.n_cells = n_cells; a.mode = limiter; a.w0 = weno_w0; a.eps = weno_eps;
   Check(LaunchDgLimiter(cell_type, degree, a, stream), "DG limiter launch");
 }

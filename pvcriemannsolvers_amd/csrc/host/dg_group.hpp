@@ -48,7 +48,7 @@ struct DgGroup {
   DeviceArray<uint8_t> flags;
   DeviceArray<double> flag_rec;        // [cell][kDgFlagRecord], the detector's view of the current state
   DeviceArray<int32_t> troubled;       // flagged cells of the last limiter pass, then their count
-  DeviceArray<double> proj, beta, eig;
+  DeviceArray<double> proj, eig;
 
   void Build(const mcfd_mesh_desc &mesh, const mcfd_scheme_desc &scheme,
              const std::vector<int32_t> &bc_kinds, const std::vector<double> &bc_states);

@@ -674,7 +674,11 @@ __device__ double Smoothness(const double *c, int stride, const double *s, const
 
 // weno.hpp:121-155 (Lazy::Borrow), then the smoothness of every slot.  Slot
 // i < n_adj projects neighbour i onto the cell's basis and shifts its average
-// onto the cell's; slot n_adj is the cell's own expansion.  A wavefront works
+// onto the cell's; slot n_adj is the cell's own expansion.  weno::Eigen
+// takes the projections from a.proj.  weno::Lazy (weno.hpp:157-194) blends
+// them here, in LDS, and stages the result and its Taylor coefficients at the
+// cell's list position; DgWenoScatterKernel writes them over the state once
+// every projection has read its neighbours.  A wavefront works
 // on CPW troubled cells at once, LPC = 64 / CPW lanes each: the per-point and
 // per-(slot, component) phases of a tetrahedron (14 points, 25 smoothness
 // tasks) would leave most of a 64-lane wavefront idle.  Every lane runs the
@@ -695,6 +699,8 @@ __global__ __launch_bounds__(64) void DgWenoProjectKernel(DgLimiterArgs a) {
   __shared__ double s_pr[CPW][NA + 1][5 * N];  // projections, the cell's own last
   __shared__ double s_basis[CPW][PACKED];   // the cell's S
   __shared__ double s_vpow[CPW][2];         // its smoothness volume scales
+  __shared__ double s_wt[CPW][NA + 1][5];   // weno::Lazy: nonlinear weight of each projection
+  __shared__ double s_fresh[CPW][5 * N];    // and the blended expansion
   const int g = threadIdx.x / LPC, lane = threadIdx.x - (threadIdx.x / LPC) * LPC;
   const DgTables *tb = a.tables;
   const int count = *a.n_troubled;
@@ -757,68 +763,66 @@ __global__ __launch_bounds__(64) void DgWenoProjectKernel(DgLimiterArgs a) {
         acc += (a.u[c * 5 * N + k] * s00 - acc * s00) / s00;
       }
       s_pr[g][sl][r] = acc;
-      a.proj[(c * kLimiterSlots + sl) * 5 * N + r] = acc;
+      if (a.mode >= 3) a.proj[(c * kLimiterSlots + sl) * 5 * N + r] = acc;
     }
     for (int r = lane; on && r < 5 * N; r += LPC) {
       const double v = a.u[c * 5 * N + r];
       s_pr[g][na][r] = v;
-      a.proj[(c * kLimiterSlots + na) * 5 * N + r] = v;
+      if (a.mode >= 3) a.proj[(c * kLimiterSlots + na) * 5 * N + r] = v;
     }
     if (a.mode < 3) {  // weno::Eigen measures smoothness in characteristic variables
       __syncthreads();
+      // weno.hpp:157-194: the nonlinear weight of each (slot, component); the
+      // cell's own linear weight is 1 - n_adj w0
       for (int e = lane; on && e < (na + 1) * 5; e += LPC) {
         const int sl = e / 5, k = e - sl * 5;
-        a.beta[(c * kLimiterSlots + sl) * 5 + k] = Smoothness<kHex, P>(s_pr[g][sl] + k, 5, s, gc, tb, s_vpow[g], s_y[g],
-                                                                    s_w[g]);
+        double bb = Smoothness<kHex, P>(s_pr[g][sl] + k, 5, s, gc, tb, s_vpow[g], s_y[g], s_w[g]);
+        double wi = a.w0;
+        if (sl == na) { wi *= -na; wi += 1.0; }
+        bb += a.eps;
+        bb *= bb;
+        s_wt[g][sl][k] = wi / bb;
+      }
+      __syncthreads();
+      // the weighted sum, one lane per (coefficient, component)
+      double *staged = a.proj + (int64_t)li * 2 * 5 * N;
+      for (int r = lane; on && r < 5 * N; r += LPC) {
+        const int k = r - (r / 5) * 5;
+        double sum = 0.0;
+        for (int sl = 0; sl <= na; ++sl) sum += s_wt[g][sl][k];
+        double v = s_pr[g][na][r] * (s_wt[g][na][k] / sum);
+        for (int sl = 0; sl < na; ++sl) v += s_pr[g][sl][r] * (s_wt[g][sl][k] / sum);
+        s_fresh[g][r] = v;
+        staged[r] = v;
+      }
+      __syncthreads();
+      // its Taylor coefficients C S
+      for (int r = lane; on && r < 5 * N; r += LPC) {
+        const int j = r / 5, k = r - (r / 5) * 5;
+        double acc = 0.0;
+        for (int m = j; m < N; ++m) acc = __fma_rn(s_fresh[g][5 * m + k], s[m * (m + 1) / 2 + j], acc);
+        staged[5 * N + r] = acc;
       }
     }
     __syncthreads();  // the next cells reuse the LDS buffers
   }
 }
 
-// weno.hpp:157-194 (Lazy::Reconstruct): nonlinear weights and the weighted
-// sum, written over the state, with its Taylor coefficients, only after every
-// projection has been read.  A 256-thread workgroup takes 256 / (5 N)
-// troubled cells, one thread per (cell, coefficient, component), so the
-// projections, the state and the coefficients move as contiguous rows; each
-// thread forms the weights of its component itself (the same arithmetic for
-// all the component's threads), and the coefficients C S go through LDS.
+// weno::Lazy write-back: the staged expansion and Taylor coefficients of
+// each troubled cell over its state, after every projection has read the
+// old states (the Jacobi order of limiter/reconstruct.hpp:30-40).  One thread
+// per (list position, row double), striding over the list.
 template <int P>
-__global__ __launch_bounds__(256) void DgWenoCombineKernel(DgLimiterArgs a) {
-  constexpr int N = P == 1 ? 4 : 10, PACKED = N * (N + 1) / 2;
-  constexpr int ROW = 5 * N, CPB = 256 / ROW;  // cells per workgroup
-  __shared__ double s_fresh[CPB][ROW];
-  const int cs = threadIdx.x / ROW, r = threadIdx.x - (threadIdx.x / ROW) * ROW;
-  const int i = r / 5, k = r - (r / 5) * 5;
-  const int64_t li = (int64_t)blockIdx.x * CPB + cs;
-  const bool on = cs < CPB && li < *a.n_troubled;
-  const int64_t c = on ? a.troubled[li] : 0;
-  if (on) {
-    const int na = a.n_adj[c];
-    double wt[kLimiterSlots];
-    for (int sl = 0; sl <= na; ++sl) {
-      double wi = a.w0;
-      if (sl == na) { wi *= -na; wi += 1.0; }
-      double bb = a.beta[(c * kLimiterSlots + sl) * 5 + k];
-      bb += a.eps;
-      bb *= bb;
-      wt[sl] = wi / bb;
-    }
-    double sum = 0.0;
-    for (int sl = 0; sl <= na; ++sl) sum += wt[sl];
-    const double *pr = a.proj + c * kLimiterSlots * ROW;
-    double v = pr[na * ROW + r] * (wt[na] / sum);
-    for (int sl = 0; sl < na; ++sl) v += pr[sl * ROW + r] * (wt[sl] / sum);
-    a.u[c * ROW + r] = v;
-    s_fresh[cs][r] = v;
-  }
-  __syncthreads();
-  if (on) {
-    const double *sc = a.basis + c * PACKED;
-    const int j = i;  // this thread's Taylor coefficient
-    double acc = 0.0;
-    for (int m = j; m < N; ++m) acc = __fma_rn(s_fresh[cs][5 * m + k], sc[m * (m + 1) / 2 + j], acc);
-    a.ct[c * ROW + r] = acc;
+__global__ __launch_bounds__(256) void DgWenoScatterKernel(DgLimiterArgs a) {
+  constexpr int N = P == 1 ? 4 : 10, ROW = 5 * N;
+  const int64_t total = (int64_t)*a.n_troubled * 2 * ROW;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+    const int64_t li = e / (2 * ROW);
+    const int r = (int)(e - li * 2 * ROW);
+    const int64_t c = a.troubled[li];
+    const double v = a.proj[e];
+    if (r < ROW) a.u[c * ROW + r] = v;
+    else a.ct[c * ROW + r - ROW] = v;
   }
 }
 
@@ -1040,7 +1044,7 @@ hipError_t LaunchDgLimiter(int cell_type, int degree, const DgLimiterArgs &a, hi
   // one 64-thread workgroup per troubled cell at a time, striding over the list
   const int64_t b_proj = std::min<int64_t>(a.n_cells, 8192);
   const int64_t b_comb = (a.n_cells * 5 + 255) / 256;  // Eigen combine: one thread per (cell, component)
-  const int64_t b_lazy = (a.n_cells + 256 / (5 * n) - 1) / (256 / (5 * n));  // Lazy combine: 256 / (5 N) cells
+  const int64_t b_lazy = std::min<int64_t>((a.n_cells * 2 * 5 * n + 255) / 256, 4096);  // Lazy write-back
   if (b_comb > 0x7fffffff) return hipErrorInvalidValue;
   const int64_t b_eig = (a.n_cells * 30 + 255) / 256;
   if (b_eig > 0x7fffffff) return hipErrorInvalidValue;
@@ -1055,7 +1059,7 @@ hipError_t LaunchDgLimiter(int cell_type, int degree, const DgLimiterArgs &a, hi
     hipLaunchKernelGGL((DgWenoProjectKernel<false, P>), dim3((unsigned)b_proj), dim3(64), 0, stream, a); \
   }                                                                                                     \
   if (!eigen) {                                                                                         \
-    hipLaunchKernelGGL(DgWenoCombineKernel<P>, dim3((unsigned)b_lazy), dim3(256), 0, stream, a);        \
+    hipLaunchKernelGGL(DgWenoScatterKernel<P>, dim3((unsigned)b_lazy), dim3(256), 0, stream, a);        \
   } else {                                                                                              \
     if (hex) {                                                                                          \
       hipLaunchKernelGGL((DgWenoEigenKernel<true, P>), dim3((unsigned)b_eig), dim3(256), 0, stream, a); \

@@ -105,8 +105,9 @@ struct DgLimiterArgs {
   uint8_t *flags;         // [cell]
   int32_t *troubled;      // [cell] list of the flagged cells, in no particular order
   int32_t *n_troubled;    // its length
-  double *proj;           // scratch [cell][7][N][5]: neighbour projections, then the cell's own
-  double *beta;           // scratch [cell][7][5]: their smoothness indicators (Lazy)
+  double *proj;           // scratch, n_cells * 7 * 5 N doubles.  weno::Eigen: [cell][7][N][5] neighbour
+                          // projections, then the cell's own.  weno::Lazy: [troubled-list position][2][N][5]
+                          // the reconstructed state and its Taylor coefficients, until written back
   double *eig;            // scratch [cell][6][5][N]: blended characteristic fields per face (Eigen)
   int64_t n_cells;
   int32_t mode;           // MCFD_LIMITER_*: 1, 2 weno::Lazy on flagged / all cells; 3, 4 weno::Eigen
