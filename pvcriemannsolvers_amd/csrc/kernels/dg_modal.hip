@@ -633,6 +633,9 @@ __device__ double Smoothness(const double *c, int stride, const double *s, const
     double acc = 0.0;
     for (int i = j; i < N; ++i) acc = __fma_rn(c[stride * i], s[i * (i + 1) / 2 + j], acc);
     tc[j] = acc;
+    // one column of S in flight at a time: scheduled all at once, the 55
+    // basis loads would hold twice the registers of the rest of the kernel
+    asm volatile("" ::: "memory");
   }
   double integ[10];
 #pragma unroll
@@ -685,23 +688,30 @@ __device__ double Smoothness(const double *c, int stride, const double *s, const
 // same number of loop trips and barriers; a lane group past the end of the
 // list only skips the work.
 template <bool kHex, int P>
-__global__ __launch_bounds__(64) void DgWenoProjectKernel(DgLimiterArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void DgWenoProjectKernel(DgLimiterArgs a) {
   constexpr int N = P == 1 ? 4 : 10, PACKED = N * (N + 1) / 2, NQ = DgShape<kHex, P>::kNQ;
   constexpr int NA = kHex ? 6 : 4;          // neighbours at most
   constexpr int CPW = kHex ? 1 : 2;         // troubled cells per wavefront
   constexpr int LPC = 64 / CPW;             // lanes per cell
-  __shared__ double s_ct[CPW][NA][5 * N];   // neighbours' Taylor coefficients
+  // Buffers whose lives do not overlap share LDS (12.7 KB per workgroup for
+  // tetrahedra, 3 waves per SIMD): the neighbours' Taylor coefficients are
+  // dead once their values at the quadrature points are formed, and those
+  // values once the projections are.
+  __shared__ double s_cp[CPW][NA + 1][5 * N];   // neighbours' Taylor coefficients; then the projections, own last
+  __shared__ double s_fv[CPW][NA * NQ * 5];     // neighbour polynomials at the points; then weights and blend
   __shared__ double s_shift[CPW][NA][3];    // own centre - neighbour centre
   __shared__ double s_y[CPW][NQ][3];        // the cell's quadrature points, offsets from its centre
   __shared__ double s_phi[CPW][NQ][N];      // its basis at them
   __shared__ double s_w[CPW][NQ];           // and their global weights
-  __shared__ double s_f[CPW][NA][NQ][5];    // neighbour polynomials at them
-  __shared__ double s_pr[CPW][NA + 1][5 * N];  // projections, the cell's own last
   __shared__ double s_basis[CPW][PACKED];   // the cell's S
   __shared__ double s_vpow[CPW][2];         // its smoothness volume scales
-  __shared__ double s_wt[CPW][NA + 1][5];   // weno::Lazy: nonlinear weight of each projection
-  __shared__ double s_fresh[CPW][5 * N];    // and the blended expansion
+  static_assert((NA + 1) * 5 + 5 * N <= NA * NQ * 5, "weights and blend fit the point values' buffer");
   const int g = threadIdx.x / LPC, lane = threadIdx.x - (threadIdx.x / LPC) * LPC;
+  double (*const s_ct)[5 * N] = s_cp[g];    // [NA][5 N], until the point values are formed
+  double (*const s_pr)[5 * N] = s_cp[g];    // [NA + 1][5 N], from the projections on
+  double (*const s_f)[NQ][5] = reinterpret_cast<double (*)[NQ][5]>(s_fv[g]);  // [NA][NQ][5]
+  double (*const s_wt)[5] = reinterpret_cast<double (*)[5]>(s_fv[g]);         // weno::Lazy: [NA + 1][5] weights
+  double *const s_fresh = s_fv[g] + (NA + 1) * 5;                             // and the blend [5 N]
   const DgTables *tb = a.tables;
   const int count = *a.n_troubled;
   for (int base = blockIdx.x * CPW; base < count; base += gridDim.x * CPW) {
@@ -713,7 +723,7 @@ __global__ __launch_bounds__(64) void DgWenoProjectKernel(DgLimiterArgs a) {
     const double *s = s_basis[g];
     for (int e = lane; e < na * 5 * N; e += LPC) {
       const int sl = e / (5 * N), r = e - sl * (5 * N);
-      s_ct[g][sl][r] = a.ct[(int64_t)a.adj[c * 6 + sl] * 5 * N + r];
+      s_ct[sl][r] = a.ct[(int64_t)a.adj[c * 6 + sl] * 5 * N + r];
     }
     for (int e = lane; on && e < PACKED; e += LPC) s_basis[g][e] = a.basis[c * PACKED + e];
     if (on && lane < 2) s_vpow[g][lane] = a.flag_rec[c * kDgFlagRecord + kDgFlagScale1 + lane];
@@ -737,6 +747,7 @@ __global__ __launch_bounds__(64) void DgWenoProjectKernel(DgLimiterArgs a) {
         double phi = 0.0;
         for (int j = 0; j <= i; ++j) phi = __fma_rn(s[i * (i + 1) / 2 + j], t[j], phi);
         s_phi[g][q][i] = phi;
+        asm volatile("" ::: "memory");  // one row of S in flight at a time
       }
 #pragma unroll
       for (int d = 0; d < 3; ++d) s_y[g][q][d] = y[d];
@@ -747,9 +758,9 @@ __global__ __launch_bounds__(64) void DgWenoProjectKernel(DgLimiterArgs a) {
       const int sl = e / NQ, q = e - sl * NQ;
       const double yo[3] = { s_y[g][q][0] + s_shift[g][sl][0], s_y[g][q][1] + s_shift[g][sl][1],
                              s_y[g][q][2] + s_shift[g][sl][2] };
-      const State f = EvalTaylor<P>(s_ct[g][sl], yo);
+      const State f = EvalTaylor<P>(s_ct[sl], yo);
 #pragma unroll
-      for (int k = 0; k < 5; ++k) s_f[g][sl][q][k] = f.v[k];
+      for (int k = 0; k < 5; ++k) s_f[sl][q][k] = f.v[k];
     }
     __syncthreads();
     // projection.hpp:28-43 (Project) and ProjectionWrapper::operator+=(my
@@ -757,17 +768,17 @@ __global__ __launch_bounds__(64) void DgWenoProjectKernel(DgLimiterArgs a) {
     for (int e = lane; e < na * 5 * N; e += LPC) {
       const int sl = e / (5 * N), r = e - sl * (5 * N), i = r / 5, k = r - (r / 5) * 5;
       double acc = 0.0;
-      for (int q = 0; q < NQ; ++q) acc += (s_f[g][sl][q][k] * s_phi[g][q][i]) * s_w[g][q];
+      for (int q = 0; q < NQ; ++q) acc += (s_f[sl][q][k] * s_phi[g][q][i]) * s_w[g][q];
       if (i == 0) {
         const double s00 = s[0];
         acc += (a.u[c * 5 * N + k] * s00 - acc * s00) / s00;
       }
-      s_pr[g][sl][r] = acc;
+      s_pr[sl][r] = acc;
       if (a.mode >= 3) a.proj[(c * kLimiterSlots + sl) * 5 * N + r] = acc;
     }
     for (int r = lane; on && r < 5 * N; r += LPC) {
       const double v = a.u[c * 5 * N + r];
-      s_pr[g][na][r] = v;
+      s_pr[na][r] = v;
       if (a.mode >= 3) a.proj[(c * kLimiterSlots + na) * 5 * N + r] = v;
     }
     if (a.mode < 3) {  // weno::Eigen measures smoothness in characteristic variables
@@ -776,12 +787,12 @@ __global__ __launch_bounds__(64) void DgWenoProjectKernel(DgLimiterArgs a) {
       // cell's own linear weight is 1 - n_adj w0
       for (int e = lane; on && e < (na + 1) * 5; e += LPC) {
         const int sl = e / 5, k = e - sl * 5;
-        double bb = Smoothness<kHex, P>(s_pr[g][sl] + k, 5, s, gc, tb, s_vpow[g], s_y[g], s_w[g]);
+        double bb = Smoothness<kHex, P>(s_pr[sl] + k, 5, s, gc, tb, s_vpow[g], s_y[g], s_w[g]);
         double wi = a.w0;
         if (sl == na) { wi *= -na; wi += 1.0; }
         bb += a.eps;
         bb *= bb;
-        s_wt[g][sl][k] = wi / bb;
+        s_wt[sl][k] = wi / bb;
       }
       __syncthreads();
       // the weighted sum, one lane per (coefficient, component)
@@ -789,10 +800,10 @@ __global__ __launch_bounds__(64) void DgWenoProjectKernel(DgLimiterArgs a) {
       for (int r = lane; on && r < 5 * N; r += LPC) {
         const int k = r - (r / 5) * 5;
         double sum = 0.0;
-        for (int sl = 0; sl <= na; ++sl) sum += s_wt[g][sl][k];
-        double v = s_pr[g][na][r] * (s_wt[g][na][k] / sum);
-        for (int sl = 0; sl < na; ++sl) v += s_pr[g][sl][r] * (s_wt[g][sl][k] / sum);
-        s_fresh[g][r] = v;
+        for (int sl = 0; sl <= na; ++sl) sum += s_wt[sl][k];
+        double v = s_pr[na][r] * (s_wt[na][k] / sum);
+        for (int sl = 0; sl < na; ++sl) v += s_pr[sl][r] * (s_wt[sl][k] / sum);
+        s_fresh[r] = v;
         staged[r] = v;
       }
       __syncthreads();
@@ -800,7 +811,7 @@ __global__ __launch_bounds__(64) void DgWenoProjectKernel(DgLimiterArgs a) {
       for (int r = lane; on && r < 5 * N; r += LPC) {
         const int j = r / 5, k = r - (r / 5) * 5;
         double acc = 0.0;
-        for (int m = j; m < N; ++m) acc = __fma_rn(s_fresh[g][5 * m + k], s[m * (m + 1) / 2 + j], acc);
+        for (int m = j; m < N; ++m) acc = __fma_rn(s_fresh[5 * m + k], s[m * (m + 1) / 2 + j], acc);
         staged[5 * N + r] = acc;
       }
     }
