@@ -192,16 +192,20 @@ def test_limiter_flags_and_state_match_oracle(mode, name):
 
 
 @pytest.mark.parametrize("mode", ["weno_all", "weno_eigen_all"])
-def test_limiter_on_every_cell_of_13824_tets_matches_oracle(mode):
-    """Every cell of 24x24x4 hexahedra x 6 tetrahedra reconstructed: the troubled list (13 824 cells) is longer than
-    one pass of the projection grid (6144 workgroups x 2 cells) and of the write-back grid (4096 x 256 threads for
-    13 824 x 100 doubles), so both grid-stride loops take a second trip."""
-    mesh = meshgen.box_tet(24, 24, 4, hi=(1.0, 1.0, 0.25), jitter=0.15, seed=8)
+@pytest.mark.parametrize("cells", ["tets", "hexes"])
+def test_limiter_on_every_cell_at_scale_matches_oracle(mode, cells):
+    """Every cell reconstructed, with a troubled list longer than one pass of the projection grid (6144 workgroups of
+    2 tetrahedra or 1 hexahedron): 24x24x4 hexahedra x 6 = 13 824 tetrahedra, or 20^3 = 8000 hexahedra.  The
+    tetrahedra also take a second trip of the write-back grid (4096 x 256 threads for 13 824 x 100 doubles)."""
+    if cells == "tets":
+        mesh, n = meshgen.box_tet(24, 24, 4, hi=(1.0, 1.0, 0.25), jitter=0.15, seed=8), 13824
+    else:
+        mesh, n = meshgen.box_hex(20, 20, 20, hi=(1.0, 1.0, 1.0), shuffle_seed=9), 8000
     bcs = [("inviscid_wall", None)] * 6
     o = ol.DgOracle(mesh, degree=2, riemann=1, bcs=[(1, [0] * 5)] * 6)
     o.set_limiter(LIMITER_MODES[mode])
     g = spatial.FiniteElement(mesh, degree=2, scheme="dg", riemann="hllc", bcs=bcs, limiter=mode)
-    assert g.n_cells == 13824
+    assert g.n_cells == n
     u = o.approximate(quadrant_state(o.cells()["xyz"], seed=23))
     g.set_solution_column(u)
     v, flags = o.limit(u)
