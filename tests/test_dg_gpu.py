@@ -191,6 +191,21 @@ def test_limiter_flags_and_state_match_oracle(mode, name):
     g.close()
 
 
+@pytest.mark.parametrize("mode", ["weno", "weno_eigen"])
+def test_limiter_without_troubled_cells_keeps_the_state(mode):
+    """A uniform state flags nothing: the projection and write-back loops take no trip and the column is returned
+    bit for bit."""
+    mesh = MESHES["tets"]()
+    state = cons(1.0, (0.3, -0.2, 0.1), 1.0)
+    g = spatial.FiniteElement(mesh, degree=2, scheme="dg", riemann="hllc", bcs=[("smart", state)] * 6, limiter=mode)
+    o = ol.DgOracle(mesh, degree=2, riemann=1, bcs=[(0, list(state))] * 6)
+    u = o.approximate(np.broadcast_to(state, o.cells()["xyz"].shape[:-1] + (5,)))
+    g.set_solution_column(u)
+    assert not g.limiter_flags().any()
+    np.testing.assert_array_equal(g.get_solution_column(), u)
+    g.close()
+
+
 @pytest.mark.parametrize("mode", ["weno_all", "weno_eigen_all"])
 @pytest.mark.parametrize("cells", ["tets", "hexes"])
 def test_limiter_on_every_cell_at_scale_matches_oracle(mode, cells):
