@@ -157,6 +157,11 @@ int mcfd_set_time(mcfd_solver *h, double t);
 int mcfd_set_solution_column(mcfd_solver *h, const double *column);
 /* System::GetSolutionColumn (ode.hpp:40, fem.hpp:237-247); host column. */
 int mcfd_get_solution_column(mcfd_solver *h, double *column);
+/* Part::ReadSolutions + ScatterSolutions, the restart path of
+ * demo/euler/sourceless.cpp:167-171 (mesh/part.hpp:1070-1208): sets the
+ * solution's coefficients as stored, without the limiter pass that
+ * mcfd_set_solution_column runs on a handle with a limiter; host column. */
+int mcfd_restore_solution_column(mcfd_solver *h, const double *column);
 /* System::GetResidualColumn (ode.hpp:47, fem.hpp:248-258 with the FR
  * det(J) scaling of fr/general.hpp:249-268); host column. */
 int mcfd_get_residual_column(mcfd_solver *h, double *column);

@@ -63,6 +63,7 @@ SIGNATURES = {
     "mcfd_set_time": [ctypes.c_void_p, ctypes.c_double],
     "mcfd_set_solution_column": [ctypes.c_void_p, _dp],
     "mcfd_get_solution_column": [ctypes.c_void_p, _dp],
+    "mcfd_restore_solution_column": [ctypes.c_void_p, _dp],
     "mcfd_get_residual_column": [ctypes.c_void_p, _dp],
     "mcfd_rk3_update": [ctypes.c_void_p, ctypes.c_double, ctypes.c_double],
     "mcfd_rk3_stage": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_double],

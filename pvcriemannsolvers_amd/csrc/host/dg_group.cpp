@@ -439,11 +439,11 @@ void DgGroup::Limit(double *u, double *ct, hipStream_t stream) {
   Check(LaunchDgLimiter(cell_type, degree, a, stream), "DG limiter launch");
 }
 
-void DgGroup::SolutionChanged(hipStream_t stream) {
+void DgGroup::SolutionChanged(hipStream_t stream, bool limit) {
   Check(LaunchDgTaylorCoefficients(degree, u0.ptr, basis.ptr, n_cells, ct0.ptr, limiter ? flag_rec.ptr : nullptr,
                                    stream),
         "DG Taylor launch");
-  Limit(u0.ptr, ct0.ptr, stream);  // spatial/with_limiter.hpp:56-59
+  if (limit) Limit(u0.ptr, ct0.ptr, stream);  // spatial/with_limiter.hpp:56-59
 }
 
 void DgGroup::Stage(int stage, double dt, hipStream_t stream, unsigned long long *dt_min_bits) {

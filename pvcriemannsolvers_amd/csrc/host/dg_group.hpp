@@ -66,8 +66,9 @@ struct DgGroup {
   void Stage(int stage, double dt, hipStream_t stream, unsigned long long *dt_min_bits = nullptr);
   // Limiter pass over the state `u` with Taylor coefficients `ct`.
   void Limit(double *u, double *ct, hipStream_t stream);
-  // The state u0 was set from outside: form its Taylor coefficients and limit it.
-  void SolutionChanged(hipStream_t stream);
+  // The state u0 was set from outside: form its Taylor coefficients (and the
+  // detector record) and, unless `limit` is false (a restored state), limit it.
+  void SolutionChanged(hipStream_t stream, bool limit = true);
   // min over cells of length / max wave speed at the first N quadrature points.
   void TimeStepBits(unsigned long long *dt_min_bits, hipStream_t stream);
   // u0 = coefficients of f given at the quadrature points ([cell][nq][5] host

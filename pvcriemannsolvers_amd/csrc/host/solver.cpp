@@ -170,6 +170,18 @@ int mcfd_set_solution_column(mcfd_solver *h, const double *column) {
   });
 }
 
+int mcfd_restore_solution_column(mcfd_solver *h, const double *column) {
+  MCFD_REQUIRE_DEVICE(h);
+  if (!column) return Fail(MCFD_EINVAL, "null column");
+  return Guard([&] {
+    h->dt_fresh = false;
+    Check(hipMemcpyAsync(h->Solution(), column, h->ColumnSize() * sizeof(double),
+                         hipMemcpyHostToDevice, h->stream), "restore solution");
+    if (h->dg) h->modal.SolutionChanged(h->stream, /*limit=*/false);
+    Check(hipStreamSynchronize(h->stream), "restore solution sync");
+  });
+}
+
 int mcfd_get_solution_column(mcfd_solver *h, double *column) {
   MCFD_REQUIRE_DEVICE(h);
   if (!column) return Fail(MCFD_EINVAL, "null column");

@@ -21,11 +21,14 @@ def write_restart(fe, path: str, t: float) -> None:
 
 def read_restart(fe, path: str) -> float:
     """Sets `fe`'s solution from a restart file written for the same layout
-    and returns its time; raises ValueError if the layout differs."""
+    and returns its time; raises ValueError if the layout differs.  The
+    column is restored as stored: no limiter pass runs on it, as the
+    reference's restart reads the coefficients back into the cells
+    (demo/euler/sourceless.cpp:167-171)."""
     with np.load(path, allow_pickle=False) as z:
         layout = (str(z["scheme"]), int(z["degree"]), int(z["n_cells"]), int(z["n_per_cell"]))
         want = (fe.scheme, fe.degree, fe.n_cells, fe.n_per_cell)
         if layout != want:
             raise ValueError(f"restart file layout {layout} does not match the solver's {want}")
-        fe.set_solution_column(z["column"])
+        fe.restore_solution_column(z["column"])
         return float(z["time"])

@@ -106,6 +106,14 @@ class FiniteElement(System):
         assert column.size == self.column_size
         C.check(C.load().mcfd_set_solution_column(self._h, C.ptr(column, C._dp)))
 
+    def restore_solution_column(self, column):
+        """Sets the solution as stored, without the limiter pass of
+        set_solution_column: the restart path (Part::ReadSolutions +
+        ScatterSolutions, demo/euler/sourceless.cpp:167-171)."""
+        column = np.ascontiguousarray(column, np.float64).reshape(-1)
+        assert column.size == self.column_size
+        C.check(C.load().mcfd_restore_solution_column(self._h, C.ptr(column, C._dp)))
+
     def get_solution_column(self):
         out = np.empty(self.column_size)
         C.check(C.load().mcfd_get_solution_column(self._h, C.ptr(out, C._dp)))
