@@ -12,6 +12,7 @@
 #ifndef MINI_SPATIAL_MI355X_HPP_
 #define MINI_SPATIAL_MI355X_HPP_
 
+#include <algorithm>
 #include <array>
 #include <cstdint>
 #include <stdexcept>
@@ -56,7 +57,7 @@ class FiniteElement : public temporal::System<double> {
   FiniteElement &operator=(FiniteElement const &) = delete;
   FiniteElement(FiniteElement &&that) noexcept
       : h_(std::exchange(that.h_, nullptr)), n_cells_(that.n_cells_), n_per_cell_(that.n_per_cell_),
-        size_(that.size_), degree_(that.degree_), cell_type_(that.cell_type_),
+        size_(that.size_), degree_(that.degree_), cell_type_(that.cell_type_), lumped_lobatto_(that.lumped_lobatto_),
         field_names_(std::move(that.field_names_)) {}
   ~FiniteElement() override {
     if (h_) mcfd_destroy(h_);
@@ -73,6 +74,17 @@ class FiniteElement : public temporal::System<double> {
     s.c_next = c_next;
     s.precision = precision;
     return FiniteElement(mesh, s, bcs);
+  }
+  // fr::Lobatto<Part, rotated::Euler<Riemann>> (spatial/fr/lobatto.hpp): FR
+  // on Lobatto nodes with the lumped correction c_next = p / (2p + 1), and
+  // its doubled time step
+  template <class Riemann>
+  static FiniteElement FluxReconstructionLobatto(mcfd_mesh_desc const &mesh, int degree,
+                                                 std::vector<mcfd_bc_desc> const &bcs) {
+    FiniteElement fe = FluxReconstruction<Riemann>(mesh, degree, bcs, MCFD_NODES_LOBATTO,
+                                                   degree / (2.0 * degree + 1.0));
+    fe.lumped_lobatto_ = true;
+    return fe;
   }
   // dg::General<Part<..., Projection<double, 3, p, 5>>, rotated::Euler<Riemann>>
   template <class Riemann>
@@ -123,11 +135,34 @@ class FiniteElement : public temporal::System<double> {
   }
 
   // ---- the rest of the FiniteElement surface used by the drivers
-  // GetTimeStep(dt_guess, rk_order) of fem.hpp:417-435 with an explicit CFL
-  double GetTimeStep(double cfl, double dt_guess = 1e300) const {
+  // FiniteElement::GetTimeStep(dt_guess, rk_order) of fem.hpp:412-435:
+  // min(dt_guess, CFL * min over cells of length / max |u| + c), with the CFL
+  // number of the reference's table for this degree; fr::Lobatto (a handle
+  // from FluxReconstructionLobatto) returns min(dt_guess, 2 dt)
+  // (fr/lobatto.hpp:220-222).  Entries the table leaves empty give 0, as
+  // they do in the reference.
+  double GetTimeStep(double dt_guess, int rk_order) const {
+    const double dt = GetTimeStepForCfl(CflNumber(rk_order, degree_), dt_guess);
+    return lumped_lobatto_ ? std::min(dt_guess, 2.0 * dt) : dt;
+  }
+  // The same reduction with an explicit CFL number.
+  double GetTimeStepForCfl(double cfl, double dt_guess = 1e300) const {
     double dt = 0;
     Check(mcfd_get_time_step(h_, cfl, dt_guess, &dt));
     return dt;
+  }
+  // cfl_[rk_order][degree] of fem.hpp:26-37
+  static double CflNumber(int rk_order, int degree) {
+    static const double kCfl[5][9] = {
+      { 1.000 },
+      { 1.000, 0.333 },
+      { 1.000, 0.333, 0.209, 0.130, 0.089, 0.066, 0.051, 0.040, 0.033 },
+      { 1.256, 0.409, 0.209, 0.130, 0.089, 0.066, 0.051, 0.040, 0.033 },
+      { 1.392, 0.464, 0.235, 0.145, 0.100, 0.073, 0.056, 0.045, 0.037 },
+    };
+    if (rk_order < 0 || rk_order > 4 || degree < 0 || degree > 8)
+      throw std::invalid_argument("no CFL number for this RK order and degree");
+    return kCfl[rk_order][degree];
   }
   // Approximate(f) of fem.hpp:147-154: values of f at SamplePoints()
   std::vector<double> SamplePoints() const {
@@ -182,6 +217,7 @@ class FiniteElement : public temporal::System<double> {
   int64_t size_ = 0;
   int degree_ = 0;
   int32_t cell_type_ = MCFD_HEXA_8;
+  bool lumped_lobatto_ = false;  // fr::Lobatto
   // demo/euler/sourceless.cpp:109-110
   std::array<std::string, 5> field_names_{ "Density", "MomentumX", "MomentumY", "MomentumZ",
                                            "EnergyStagnationDensity" };

@@ -88,7 +88,7 @@ bool Compare(char const *name, FiniteElement &system_path, FiniteElement &fused_
              bool limited) {
   system_path.Approximate(Values(system_path, jump));
   fused_path.Approximate(Values(fused_path, jump));
-  const double dt = 0.5 * system_path.GetTimeStep(0.1);
+  const double dt = 0.5 * system_path.GetTimeStepForCfl(0.1);
   mini::temporal::RungeKutta<3, double> rk;
   for (int s = 0; s < steps; ++s) {
     rk.Update(&system_path, s * dt, dt);
@@ -107,9 +107,9 @@ bool Compare(char const *name, FiniteElement &system_path, FiniteElement &fused_
   // setting a column limits it again, so the check does not apply)
   bool dt_same = true;
   if (!limited) {
-    const double dt_fused = fused_path.GetTimeStep(0.1);
+    const double dt_fused = fused_path.GetTimeStepForCfl(0.1);
     fused_path.SetSolutionColumn(b);
-    dt_same = fused_path.GetTimeStep(0.1) == dt_fused;
+    dt_same = fused_path.GetTimeStepForCfl(0.1) == dt_fused;
   }
   const bool ok = differ == 0 && finite && dt_same && a.size() > 0;
   std::printf("%s %s: %zu of %zu values differ (max |diff| %.3e), time step %s\n", ok ? "PASS" : "FAIL", name, differ,
@@ -182,6 +182,22 @@ int main() {
       std::printf("%s average::Reconstruct leaves cells constant (%d), majority::Reconstruct keeps them (%d)\n",
                   constant && kept ? "PASS" : "FAIL", (int)constant, (int)kept);
       ok &= constant && kept;
+    }
+    {
+      // GetTimeStep(dt_guess, rk_order) of fem.hpp:417-435 with the cfl_ table,
+      // and fr::Lobatto's doubled step (fr/lobatto.hpp:220-222)
+      auto gl = FiniteElement::FluxReconstruction<Roe>(walls.desc, 3, wall_bcs);
+      auto lob = FiniteElement::FluxReconstructionLobatto<Roe>(walls.desc, 3, wall_bcs);
+      gl.Approximate(Values(gl, false));
+      lob.Approximate(Values(lob, false));
+      const double g3 = gl.GetTimeStep(1e300, 3), l3 = lob.GetTimeStep(1e300, 3);
+      const double base = lob.GetTimeStepForCfl(0.130);
+      bool same = g3 == gl.GetTimeStepForCfl(0.130) && l3 == 2.0 * base && g3 > 0;
+      same &= gl.GetTimeStep(1e300, 4) == gl.GetTimeStepForCfl(0.145);
+      same &= gl.GetTimeStep(0.5 * g3, 3) == 0.5 * g3 && lob.GetTimeStep(1.5 * base, 3) == 1.5 * base;
+      std::printf("%s GetTimeStep(dt_guess, rk_order): cfl_[3][3] = 0.130 (%.6e), fr::Lobatto twice (%.6e)\n",
+                  same ? "PASS" : "FAIL", g3, l3);
+      ok &= same;
     }
     // errors of the C ABI surface as exceptions
     bool threw = false;
