@@ -7,16 +7,21 @@
 // and WithLimiter (spatial/with_limiter.hpp:40-66).  Columns keep the
 // reference layout column[(cell * N + i) * 5 + k].  Boundary conditions are
 // given per boundary tag at creation (constant states, the SetSmartBoundary /
-// SetInviscidWall / SetSupersonic* registrations of fem.hpp:196-222).  Errors
-// of the C ABI are thrown as std::runtime_error carrying its message.
+// SetInviscidWall / SetSupersonic* registrations of fem.hpp:196-222), and a
+// tag's f(xyz, t) after it (SetBoundaryFunction).  Errors of the C ABI are
+// thrown as std::runtime_error carrying its message.
 #ifndef MINI_SPATIAL_MI355X_HPP_
 #define MINI_SPATIAL_MI355X_HPP_
 
 #include <algorithm>
 #include <array>
 #include <cstdint>
+#include <functional>
+#include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -58,10 +63,12 @@ class FiniteElement : public temporal::System<double> {
   FiniteElement(FiniteElement &&that) noexcept
       : h_(std::exchange(that.h_, nullptr)), n_cells_(that.n_cells_), n_per_cell_(that.n_per_cell_),
         size_(that.size_), degree_(that.degree_), cell_type_(that.cell_type_), lumped_lobatto_(that.lumped_lobatto_),
+        bc_functions_(std::move(that.bc_functions_)),
         field_names_(std::move(that.field_names_)) {}
   ~FiniteElement() override {
     if (h_) mcfd_destroy(h_);
   }
+  using Global = std::array<double, 3>;  // a point, as the reference's Global
 
   // fr::General<Part, rotated::Euler<Riemann>> with the Vincent correction
   // c_next (1 = DG, p/(2p+1) = Huynh g2) on Gauss-Legendre or Lobatto nodes
@@ -173,6 +180,56 @@ class FiniteElement : public temporal::System<double> {
     return xyz;
   }
   void Approximate(std::vector<double> const &values) { Check(mcfd_approximate(h_, values.data())); }
+  // Approximate(func) of fem.hpp:147-154 with the reference drivers' initial
+  // condition, a callable Value(Global const &xyz) (demo/euler/riemann2d.cpp:20-25):
+  // evaluated here at SamplePoints(); its result is read as r[0..4]
+  template <class Callable, class = std::enable_if_t<std::is_invocable_v<Callable &, Global const &>>>
+  void Approximate(Callable &&func) {
+    const std::vector<double> xyz = SamplePoints();
+    const size_t n = xyz.size() / 3;
+    std::vector<double> values(n * 5);
+    for (size_t p = 0; p < n; ++p) {
+      const Global x = { xyz[3 * p], xyz[3 * p + 1], xyz[3 * p + 2] };
+      const auto v = func(x);
+      for (int k = 0; k < 5; ++k) values[5 * p + k] = v[k];
+    }
+    Approximate(values);
+  }
+
+  // ---- boundary values that vary in space and time
+  // The f(xyz, t) of SetSmartBoundary / SetSupersonicInlet / SetSubsonicInlet /
+  // SetSubsonicOutlet (fem.hpp:196-222; demo/euler/riemann2d.cpp:28-33) for
+  // the boundary tag `tag`, whose kind was given at creation: a callable
+  // Value(Global const &xyz, double t), read as r[0..4], evaluated at every
+  // point of the tag at each residual's time.  It is called from this
+  // object's methods on the calling thread and must not throw.
+  template <class Callable>
+  void SetBoundaryFunction(int32_t tag, Callable &&func) {
+    auto fn = std::make_unique<BoundaryFunction>(
+        [f = std::forward<Callable>(func)](int64_t n, double const *xyz, double t, double *states) {
+          for (int64_t i = 0; i < n; ++i) {
+            const Global x = { xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2] };
+            const auto v = f(x, t);
+            for (int k = 0; k < 5; ++k) states[5 * i + k] = v[k];
+          }
+        });
+    Check(mcfd_set_boundary_function(h_, tag, &CallBoundaryFunction, fn.get()));
+    bc_functions_[tag] = std::move(fn);
+  }
+  // A boundary function of the point only, given as its values [n][5] at
+  // BoundaryPoints(tag): evaluated once instead of at every stage time.
+  void SetBoundaryValues(int32_t tag, std::vector<double> const &states) {
+    Check(mcfd_set_boundary_values(h_, tag, states.data()));
+    bc_functions_.erase(tag);
+  }
+  // The points of boundary tag `tag`, [n][3], in the order the functions see them.
+  std::vector<double> BoundaryPoints(int32_t tag) const {
+    int64_t n = 0;
+    Check(mcfd_boundary_points(h_, tag, &n, nullptr));
+    std::vector<double> xyz((size_t)n * 3);
+    Check(mcfd_boundary_points(h_, tag, &n, xyz.data()));
+    return xyz;
+  }
   // polynomial().LocalToGlobalAndValue of every cell at n local points
   // (mesh/vtk.hpp:431-460): xyz [cell][n][3] and values [cell][n][5] of the
   // current solution, or of `column` (which a host-only handle needs)
@@ -218,6 +275,13 @@ class FiniteElement : public temporal::System<double> {
   int degree_ = 0;
   int32_t cell_type_ = MCFD_HEXA_8;
   bool lumped_lobatto_ = false;  // fr::Lobatto
+  // the boundary functions the library calls back, owned per tag (their
+  // addresses are the callbacks' user pointers)
+  using BoundaryFunction = std::function<void(int64_t, double const *, double, double *)>;
+  static void CallBoundaryFunction(int64_t n, double const *xyz, double t, double *states, void *user) noexcept {
+    (*static_cast<BoundaryFunction *>(user))(n, xyz, t, states);
+  }
+  std::map<int32_t, std::unique_ptr<BoundaryFunction>> bc_functions_;
   // demo/euler/sourceless.cpp:109-110
   std::array<std::string, 5> field_names_{ "Density", "MomentumX", "MomentumY", "MomentumZ",
                                            "EnergyStagnationDensity" };

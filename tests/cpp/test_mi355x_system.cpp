@@ -6,6 +6,7 @@
 // back and set again is unchanged.  The limiter objects' form of WithLimiter
 // equals its identifier form, and the start-up limiters average::Reconstruct
 // and majority::Reconstruct leave every cell of a nodal solution constant.
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -67,19 +68,21 @@ struct Box {
 };
 
 // conservative state of a density wave with a shear, and a jump in x for the limiter
+// the initial condition as the reference's drivers write one: Value(Global)
+std::array<double, 5> State(FiniteElement::Global const &xyz, bool jump) {
+  const double x = xyz[0], y = xyz[1], z = xyz[2];
+  double rho = 1 + 0.2 * std::sin(2 * M_PI * (x + y + z)), pr = 1.0;
+  if (jump && x > 0.5) { rho *= 0.3; pr = 0.2; }
+  const double u = 0.4, w = -0.2 * std::cos(2 * M_PI * y), s = 0.1;
+  return { rho, rho * u, rho * w, rho * s, pr / 0.4 + 0.5 * rho * (u * u + w * w + s * s) };
+}
+
 std::vector<double> Values(FiniteElement const &fe, bool jump) {
   const auto xyz = fe.SamplePoints();
   std::vector<double> v(xyz.size() / 3 * 5);
   for (size_t p = 0; p < xyz.size() / 3; ++p) {
-    const double x = xyz[3 * p], y = xyz[3 * p + 1], z = xyz[3 * p + 2];
-    double rho = 1 + 0.2 * std::sin(2 * M_PI * (x + y + z)), pr = 1.0;
-    if (jump && x > 0.5) { rho *= 0.3; pr = 0.2; }
-    const double u = 0.4, w = -0.2 * std::cos(2 * M_PI * y), s = 0.1;
-    v[5 * p] = rho;
-    v[5 * p + 1] = rho * u;
-    v[5 * p + 2] = rho * w;
-    v[5 * p + 3] = rho * s;
-    v[5 * p + 4] = pr / 0.4 + 0.5 * rho * (u * u + w * w + s * s);
+    const auto u = State({ xyz[3 * p], xyz[3 * p + 1], xyz[3 * p + 2] }, jump);
+    for (int k = 0; k < 5; ++k) v[5 * p + k] = u[k];
   }
   return v;
 }
@@ -198,6 +201,41 @@ int main() {
       std::printf("%s GetTimeStep(dt_guess, rk_order): cfl_[3][3] = 0.130 (%.6e), fr::Lobatto twice (%.6e)\n",
                   same ? "PASS" : "FAIL", g3, l3);
       ok &= same;
+    }
+    {
+      // Approximate(ic) with the drivers' Value(Global) callable equals
+      // Approximate of its values at SamplePoints(); a boundary function
+      // f(xyz, t) returning the state s, and a value table of s, equal the
+      // constant smart boundary s (fem.hpp:196-222) over two steps
+      using Global = FiniteElement::Global;
+      const double s[5] = { 1.1, 0.3, -0.1, 0.05, 2.7 }, other[5] = { 0.9, 0.0, 0.0, 0.0, 2.0 };
+      const std::vector<mcfd_bc_desc> given(6, mi::SmartBoundary(s)), placeholder(6, mi::SmartBoundary(other));
+      auto a = FiniteElement::FluxReconstruction<Roe>(walls.desc, 3, given);
+      auto b = FiniteElement::FluxReconstruction<Roe>(walls.desc, 3, placeholder);
+      auto c = FiniteElement::FluxReconstruction<Roe>(walls.desc, 3, placeholder);
+      a.Approximate(Values(a, true));
+      b.Approximate([](Global const &x) { return State(x, true); });
+      c.Approximate([](Global const &x) { return State(x, true); });
+      const bool ic_same = a.GetSolutionColumn() == b.GetSolutionColumn();
+      for (int32_t tag = 0; tag < 6; ++tag) {
+        b.SetBoundaryFunction(tag, [&s](Global const &, double) { return std::array<double, 5>{ s[0], s[1], s[2], s[3], s[4] }; });
+        const size_t n = c.BoundaryPoints(tag).size() / 3;
+        std::vector<double> table(n * 5);
+        for (size_t p = 0; p < n; ++p)
+          for (int k = 0; k < 5; ++k) table[5 * p + k] = s[k];
+        c.SetBoundaryValues(tag, table);
+      }
+      const double dt = 0.5 * a.GetTimeStep(1e300, 3);
+      for (int step = 0; step < 2; ++step) {
+        a.Rk3Update(step * dt, dt);
+        b.Rk3Update(step * dt, dt);
+        c.Rk3Update(step * dt, dt);
+      }
+      const Column ua = a.GetSolutionColumn();
+      const bool bc_same = ua == b.GetSolutionColumn() && ua == c.GetSolutionColumn();
+      std::printf("%s Approximate(Value(Global)) (%d); SetBoundaryFunction and SetBoundaryValues equal the constant "
+                  "smart boundary (%d)\n", ic_same && bc_same ? "PASS" : "FAIL", (int)ic_same, (int)bc_same);
+      ok &= ic_same && bc_same;
     }
     // errors of the C ABI surface as exceptions
     bool threw = false;

@@ -157,9 +157,10 @@ def test_vtk_writer_cpp_equals_python_writer(tmp_path, tet):
 @pytest.mark.gpu
 def test_system_contract_cpp_on_gpu():
     """FiniteElement driven by RungeKutta<3> through the System contract equals the fused device step bit for bit;
-    the limiter objects, the start-up limiters and GetTimeStep(dt_guess, rk_order) behave as the reference's."""
+    the limiter objects, the start-up limiters, GetTimeStep(dt_guess, rk_order), Approximate of a Value(Global)
+    callable and the boundary functions behave as the reference's."""
     binary = os.path.join(CPP, "_build", "test_mi355x_system")
     assert os.path.exists(binary), "the C++ drivers are built by __graft_entry__.build()"
     r = _run(binary)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert r.stdout.count("PASS") == 7 and "FAIL" not in r.stdout, r.stdout
+    assert r.stdout.count("PASS") == 8 and "FAIL" not in r.stdout, r.stdout
