@@ -362,11 +362,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-forms", action="store_true", help="skip the mixed, batched and modal-DG forms")
     ap.add_argument("--dg-only", action="store_true", help="run only the modal-DG forms (for profiling)")
+    ap.add_argument("--fr-forms-only", action="store_true",
+                    help="run only the mixed-precision and batched FR forms (for profiling)")
     ap.add_argument("--dg-t-end", type=float, default=0.25,
                     help="final time of the tetrahedral DG + WENO run (0.25 is the workload's own)")
     args = ap.parse_args()
     if args.dg_only:
         print(json.dumps({"dg_sod": bench_dg_sod(), "dg_tets_weno": bench_dg_tets(t_end=args.dg_t_end)}), flush=True)
+        return
+    if args.fr_forms_only:
+        print(json.dumps({"mixed": bench_mixed(), "batched": bench_batched()}), flush=True)
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the library has no multi-GPU path; use --gpus 1", file=sys.stderr)
